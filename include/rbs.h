@@ -1,0 +1,224 @@
+/*
+ * rbs.h -- C-ABI of librbs.so, the MI355X (gfx950) solver time-step library.
+ *
+ * The reference (BoykoNeov/river-basin-sim) has no native boundary: its solver is
+ * Python driving Warp kernels through the scheme seam
+ *     compute_dt(state, alpha, dt_max) -> float
+ *     step(state, dt, rain, rain_scale, t)            (solver/core/schemes.py:1-52)
+ * over a `State` of device arrays (solver/core/state.py:38-226).  This header is
+ * the C boundary a Python ctypes binding (or any FFI) binds in its place.  Each
+ * declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every call returns int: 0 = ok, < 0 = error; rbs_last_error() gives the text
+ *     (thread-local).  RBS_E_HIP codes carry the HIP error in the message.
+ *   - Plain pointers and sizes only.  Host buffers are borrowed for the duration of
+ *     the call.  Device memory is owned by the library.
+ *   - A state handle is not thread-safe; distinct handles may be driven from
+ *     distinct host threads.  All work of one handle is ordered on its own stream.
+ *   - Fields cross the boundary dense and row-major (y, x) in the reference's
+ *     layout (solver/core/grid.py:9-23): cells (ny, nx), x-faces (ny, nx+1),
+ *     y-faces (ny+1, nx).  The device layout is private (row-padded).
+ */
+#ifndef RBS_H
+#define RBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+#define RBS_OK 0
+#define RBS_E_ARG -1      /* bad argument (shape, size, id, order)            */
+#define RBS_E_STATE -2    /* call not valid in this state (e.g. field unarmed) */
+#define RBS_E_HIP -3      /* a HIP runtime call failed                         */
+#define RBS_E_NOMEM -4    /* device allocation failed / over budget            */
+#define RBS_E_RANGE -5    /* non-positive or non-finite dt, bad time, overflow */
+
+/* ---- field ids (solver/core/state.py:38-96, sediment.py:548-571) ----------- */
+enum rbs_field {
+  RBS_F_H = 0,        /* (ny, nx)   f32  depth                                */
+  RBS_F_Z = 1,        /* (ny, nx)   f32  bed                                  */
+  RBS_F_N = 2,        /* (ny, nx)   f32  Manning n                            */
+  RBS_F_QX = 3,       /* (ny, nx+1) f32  LI x-face discharge                  */
+  RBS_F_QY = 4,       /* (ny+1, nx) f32  LI y-face discharge                  */
+  RBS_F_HU = 5,       /* (ny, nx)   f32  HLLC x-momentum                      */
+  RBS_F_HV = 6,       /* (ny, nx)   f32  HLLC y-momentum                      */
+  RBS_F_RAIN = 7,     /* (ny, nx)   f32  rain field, m/s                      */
+  RBS_F_INFIL = 8,    /* (ny, nx)   f32  infiltration rate, m/s               */
+  RBS_F_H_COMP = 9,   /* (ny, nx)   f32  Kahan debt of areal sources          */
+  RBS_F_D50 = 10,     /* (ny, nx)   f32  grain size, m                        */
+  RBS_F_QS_INT_X = 11,  /* (ny, nx+1) f32 transport integral                  */
+  RBS_F_QS_COMP_X = 12, /* (ny, nx+1) f32 its Kahan debt                      */
+  RBS_F_QS_INT_Y = 13,  /* (ny+1, nx) f32                                     */
+  RBS_F_QS_COMP_Y = 14, /* (ny+1, nx) f32                                     */
+  RBS_F_Z0 = 15,      /* (ny, nx)   f32  pristine bed the Exner rebuild uses  */
+  RBS_F_DZ_CUM = 16,  /* (ny, nx)   f64  cumulative bed change                */
+  RBS_F_DZ_UNAPPLIED = 17, /* (ny, nx) f64 bed change the bounds refused      */
+  RBS_F_DZ_LO = 18,   /* (ny, nx)   f32  lower bound on cumulative change     */
+  RBS_F_DZ_HI = 19,   /* (ny, nx)   f32  upper bound                          */
+  RBS_F_ETA = 20,     /* (ny, nx)   f32  water surface h + z (read-only)      */
+  RBS_F_COUNT = 21
+};
+
+/* ---- arming flags (state.py:98-171, hllc.py:123-152, sediment.py:658-709) -- */
+#define RBS_ARM_COMP (1u << 0)   /* h_comp: Kahan areal sources             */
+#define RBS_ARM_RAIN (1u << 1)   /* rain field                              */
+#define RBS_ARM_INFIL (1u << 2)  /* infiltration field                      */
+#define RBS_ARM_HLLC (1u << 3)   /* momentum hu, hv + HLLC stage buffers    */
+#define RBS_ARM_SED (1u << 4)    /* d50, transport integrals, dz_cum ...    */
+
+/* ---- edges and edge types (state.py:81-96) ------------------------------- */
+#define RBS_EDGE_WEST 0
+#define RBS_EDGE_EAST 1
+#define RBS_EDGE_NORTH 2
+#define RBS_EDGE_SOUTH 3
+#define RBS_BC_CLOSED 0
+#define RBS_BC_OPEN 1
+#define RBS_BC_FIXED_STAGE 2
+
+/* ---- schemes (schemes.py:38-52) ------------------------------------------- */
+#define RBS_SCHEME_LI 0
+#define RBS_SCHEME_HLLC 1
+
+typedef struct rbs_state rbs_state;
+
+/* Device ledger (volumes in m^3; float64; each *_c is its Kahan compensation).
+ * Replaces the host float64 sums of MassLedger (massbalance.py:65-158) and the
+ * per-cell loss_cum reduction State.loss_volume (state.py:173-177). */
+typedef struct rbs_ledger {
+  double volume;            /* dx^2 * sum f64(h) of the current state          */
+  double rain_in, rain_in_c;     /* rain volume (device-driven runs only)      */
+  double point_in, point_in_c;   /* point-source volume (device injection)     */
+  double infil_out, infil_out_c; /* infiltration sink                          */
+  double edge_out, edge_out_c;   /* open-edge drain (LI) / banked edge flux (HLLC) */
+  double t;                 /* device simulated clock                          */
+  double dt;                /* dt of the next device-driven step               */
+  int64_t steps;            /* steps taken                                     */
+  float speed_max;          /* LI: max h; HLLC: max |u|,|v| + sqrt(g h)        */
+  int32_t active;           /* device-driven run: 1 while t < t_stop           */
+} rbs_ledger;
+
+/* Forcing a device-driven run evaluates per step (run.py:351-388). */
+typedef struct rbs_run_params {
+  int32_t scheme;           /* RBS_SCHEME_*                                     */
+  int32_t limit;            /* LI donor limiter (local_inertial.py:551)         */
+  double alpha;             /* [run] cfl                                        */
+  double dt_max;            /* [run] dt_max                                     */
+  double rain_m_s;          /* uniform rain rate (ignored with a rain field)    */
+  double rain_duration;     /* raining while t < rain_duration - EPS_T          */
+  int32_t rain_is_field;    /* 1: rain field scaled 1/0 by the raining flag     */
+  int32_t n_sync;           /* number of sync times                             */
+  const double *sync_times; /* sorted ascending: outputs, events, end_time ... */
+  double rain_field_sum;    /* sum f64(rain field) in m/s (run.py:276)          */
+} rbs_run_params;
+
+/* ---- lifecycle ---------------------------------------------------------------- */
+/* Last error text of the calling thread ("" if none). */
+const char *rbs_last_error(void);
+/* Library version string. */
+const char *rbs_version(void);
+/* Number of visible devices; selects nothing. */
+int rbs_device_count(int32_t *count);
+
+/* State.from_bed (state.py:179-226): allocates h, z, n, qx, qy (zeroed) on the
+ * current device with a private stream.  `arm` optionally arms fields at once. */
+int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state **out);
+int rbs_state_destroy(rbs_state *st);
+/* State.arm_source_compensation / set_infiltration / set_rain_field / arm_hllc /
+ * arm_sediment (state.py:98-171, hllc.py:123, sediment.py:658).  Idempotent per
+ * flag except RBS_ARM_SED, which refuses a second arming (sediment.py:680-684). */
+int rbs_arm(rbs_state *st, uint32_t arm);
+/* Armed flags of the state. */
+int rbs_armed(const rbs_state *st, uint32_t *arm);
+/* Device bytes held by the state (footprint for admission control). */
+int rbs_state_bytes(const rbs_state *st, uint64_t *bytes);
+
+/* wp.array(...) / .assign(...) and .numpy() on State fields (state.py:200-226).
+ * `bytes` must equal the field's dense size.  Uploading h, z or hu/hv marks the
+ * cached CFL reduction stale. */
+int rbs_upload(rbs_state *st, int32_t field, const void *host, size_t bytes);
+int rbs_download(rbs_state *st, int32_t field, void *host, size_t bytes);
+
+/* State.set_open_boundaries (state.py:113-143): edge types indexed by RBS_EDGE_*. */
+int rbs_set_boundaries(rbs_state *st, const int32_t edge_types[4]);
+/* stage curve of one fixed_stage edge, piecewise linear (boundaries.py:49-70). */
+int rbs_set_stage_curve(rbs_state *st, int32_t edge, int32_t n, const double *t,
+                        const double *level);
+
+/* ---- the scheme seam (schemes.py:1-52) ------------------------------------------ */
+/* local_inertial.compute_dt (local_inertial.py:515-544) and hllc.compute_dt
+ * (hllc.py:972-990): one device max-reduction, dt formula in float64. */
+int rbs_compute_dt(rbs_state *st, int32_t scheme, double alpha, double dt_max,
+                   double *dt_out);
+/* local_inertial.step (local_inertial.py:547-727) as one fused kernel. */
+int rbs_li_step(rbs_state *st, double dt, double rain, int32_t limit, double rain_scale,
+                double t);
+/* hllc.step (hllc.py:1272-1373): two fused SSP-RK2 stage kernels. */
+int rbs_hllc_step(rbs_state *st, double dt, double rain, double rain_scale, double t);
+
+/* ---- point sources (processes/inflow.py:116-200) ----------------------------------- */
+/* InflowInjector.__init__: sources at (rows[k], cols[k]) with piecewise-linear
+ * hydrographs knots[offsets[k] .. offsets[k+1]) of (t, Q).  Duplicate cells are
+ * rejected (inflow.py:127-139). */
+int rbs_set_inflows(rbs_state *st, int32_t count, const int32_t *rows, const int32_t *cols,
+                    const int32_t *offsets, const double *knot_t, const double *knot_q);
+/* InflowInjector.apply: midpoint discharge over [t, t+dt], compensated add; writes
+ * the float32-requested volume (m^3) the ledger banks. */
+int rbs_inflow_apply(rbs_state *st, double t, double dt, double *volume_out);
+
+/* ---- morphology (processes/morphology.py:223-307, sediment.py:383-441) ------------- */
+/* exner_update + rebuild_bed + refresh eta + clear the transport integral (its
+ * Kahan debt kept).  Bounds are RBS_F_DZ_LO / RBS_F_DZ_HI (default -inf/+inf). */
+int rbs_exner_activate(rbs_state *st, double porosity);
+
+/* ---- ledger and device-driven runs -------------------------------------------------- */
+/* MassLedger inputs: device volume, sink totals, clock (massbalance.py:109-135). */
+int rbs_ledger_read(rbs_state *st, rbs_ledger *out);
+/* Reset the device clock to t and clear the device ledger's flow totals. */
+int rbs_clock_reset(rbs_state *st, double t);
+/* MultiRateScheduler.ticks + the run loop's per-tick forcing (scheduler.py:171-246,
+ * run.py:369-388) on the device: steps are chosen by the device dt reduction and the
+ * device fill_span against `p->sync_times`, until the device clock reaches t_stop
+ * (a sync time) or max_steps were taken.  Steps are replayed from a captured HIP
+ * graph; the host does not see individual steps.  *steps_taken may be NULL. */
+int rbs_run_until(rbs_state *st, const rbs_run_params *p, double t_stop, int64_t max_steps,
+                  int64_t *steps_taken);
+/* Block until the state's stream is idle. */
+int rbs_sync(rbs_state *st);
+
+/* ---- batched ensemble (no reference counterpart: the reference runs one scenario per
+ * process, run.py:140) ------------------------------------------------------------ */
+typedef struct rbs_batch rbs_batch;
+int rbs_batch_create(int32_t capacity, rbs_batch **out);
+int rbs_batch_destroy(rbs_batch *b);
+/* Add a state to the batch with its run parameters; returns its slot. */
+int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t_stop,
+                  int32_t *slot);
+/* Remove a finished (or any) state from the batch. */
+int rbs_batch_remove(rbs_batch *b, int32_t slot);
+/* Advance every member by up to `steps` device-driven steps in batched launches;
+ * members whose clock reached their t_stop idle.  n_active receives the number of
+ * members still running. */
+int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active);
+/* Per-member status: 1 = still running, 0 = reached t_stop. */
+int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active);
+
+/* ---- measurement and tuning (no reference counterpart) ------------------------------------- */
+/* Run `reps` host-driven LI steps of `dt` (as rbs_li_step with limit on), timing each
+ * fused step kernel with HIP events recorded on the state's own stream; *avg_ms
+ * receives the mean kernel duration.  Advances the state. */
+int rbs_time_li_kernel(rbs_state *st, int32_t reps, double dt, double rain, double rain_scale,
+                       double *avg_ms);
+/* Select the LI tile variant (0 = autotuned default).  Returns RBS_E_ARG for an
+ * unknown variant.  Results are bitwise identical across variants. */
+int rbs_set_li_variant(rbs_state *st, int32_t variant);
+int rbs_li_variant_count(int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+// rbs_ctl.hip -- device clock, CFL reduction and float64 ledger.
+//
+// Replaces the per-step host round trip of the reference (compute_dt's
+// atomic_max + h_max.numpy(), local_inertial.py:515-544 / hllc.py:972-990, and
+// the scheduler's fill_span, scheduler.py:48-72) with device work: the step
+// kernels leave per-block partials, and finalize_kernel reduces them in a fixed
+// order, updates the Kahan-compensated float64 ledger (massbalance.py:39-158),
+// advances the float64 clock and -- in a device-driven run -- chooses the next dt.
+#include "rbs_device.hpp"
+
+namespace rbs {
+
+// Maximum-only reduction of the CFL speed of the current state, into partials.
+// LI: h (local_inertial.py:503-512); HLLC: max(|u|,|v|) + sqrt(g h) (hllc.py:948-966).
+// Also sums f64(h) so a prime leaves the ledger volume current.
+__global__ __launch_bounds__(kThreads) void reduce_speed_kernel(const float *h, const float *hu,
+                                                                const float *hv, int64_t pitch,
+                                                                int ny, int nx, int scheme,
+                                                                Partials part, int rows_per_blk) {
+  __shared__ double s_red[4 * (kThreads / 64)];
+  const int r0 = blockIdx.x * rows_per_blk;
+  float m = 0.0f;
+  double vol = 0.0, z0 = 0.0, z1 = 0.0;
+  for (int r = r0; r < r0 + rows_per_blk && r < ny; ++r) {
+    for (int c = threadIdx.x; c < nx; c += kThreads) {
+      const int64_t o = (int64_t)r * pitch + c;
+      const float hh = h[o];
+      vol += (double)hh;
+      if (scheme == RBS_SCHEME_LI) {
+        m = fmx(m, hh);
+      } else {
+        const float cc = sqrtf(kG * fmx(hh, 0.0f));
+        float sp = cc;
+        if (hh > kHDry) {
+          const float au = fabsf(hu[o]) / hh;
+          const float av = fabsf(hv[o]) / hh;
+          sp = fmx(au, av) + cc;
+        }
+        m = fmx(m, sp);
+      }
+    }
+  }
+  block_reduce4(m, vol, z0, z1, s_red);
+  if (threadIdx.x == 0) {
+    part.speed_max[blockIdx.x] = m;
+    part.vol[blockIdx.x] = vol;
+    part.infil[blockIdx.x] = 0.0;
+    part.edge[blockIdx.x] = 0.0;
+  }
+}
+
+// Host-driven step: write the scalars the step kernels read.
+__global__ void set_scalars_kernel(DevCtl *ctl, double t, double dt, float rain, float scale) {
+  ctl->t = t;
+  ctl->dt = dt;
+  ctl->rain = rain;
+  ctl->rain_scale = scale;
+  ctl->active = 1;
+}
+
+// Next step of a device-driven run (scheduler.py:171-229 + run.py:378-388).
+__device__ void plan_next(DevCtl *ctl, const double *sync, double dx) {
+  const double t = ctl->t;
+  if (!(t < ctl->t_stop - kEpsT) || ctl->steps >= ctl->step_limit) {
+    ctl->active = 0;
+    return;
+  }
+  double dt_raw;
+  const double sm = (double)ctl->speed_max;
+  if (ctl->run_scheme == RBS_SCHEME_LI) {
+    // local_inertial.py:540-544
+    if (sm <= kHDry64) {
+      dt_raw = ctl->dt_max;
+    } else {
+      dt_raw = ctl->alpha * dx / sqrt(kGravity * sm);
+      dt_raw = dt_raw < ctl->dt_max ? dt_raw : ctl->dt_max;
+    }
+  } else {
+    // hllc.py:987-990
+    if (sm <= 0.0) {
+      dt_raw = ctl->dt_max;
+    } else {
+      dt_raw = ctl->alpha * dx / sm;
+      dt_raw = dt_raw < ctl->dt_max ? dt_raw : ctl->dt_max;
+    }
+  }
+  int k = ctl->sync_idx;
+  while (k < ctl->n_sync && !(sync[k] > t + kEpsT)) ++k;  // next_sync (scheduler.py:165-168)
+  ctl->sync_idx = k;
+  const double ns = k < ctl->n_sync ? sync[k] : INFINITY;
+  const double dt = fill_span_dev(ns - t, dt_raw);
+  if (!(dt_raw > 0.0) || !(dt > 0.0)) {  // scheduler.py:221-227: fail loudly
+    ctl->active = -1;
+    return;
+  }
+  ctl->dt = dt;
+  const bool raining = t < ctl->rain_duration - kEpsT;  // run.py:378
+  ctl->rain = ctl->run_field ? 0.0f : (raining ? (float)ctl->rain_m_s : 0.0f);
+  ctl->rain_scale = raining ? 1.0f : 0.0f;
+  ctl->active = 1;
+}
+
+// Reduce the step's (or prime's) partials in fixed order and update clock + ledger.
+// mode: FIN_PRIME (no step was taken), FIN_EXPLICIT (host-driven step),
+// FIN_DRIVEN (device-driven step; plans the next one).
+__global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partials part, int nblk,
+                                                            double area, double n_cells,
+                                                            const double *sync, double dx,
+                                                            int mode) {
+  __shared__ double s_red[4 * (kThreads / 64)];
+  if (mode == FIN_DRIVEN && ctl->active != 1) return;
+  float m = 0.0f;
+  double vol = 0.0, inf = 0.0, edge = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kThreads) {
+    m = fmx(m, part.speed_max[b]);
+    vol += part.vol[b];
+    inf += part.infil[b];
+    edge += part.edge[b];
+  }
+  block_reduce4(m, vol, inf, edge, s_red);
+  if (threadIdx.x != 0) return;
+  ctl->speed_max = m;
+  ctl->stale = 0;
+  ctl->volume = vol * area;
+  if (mode == FIN_PRIME) return;
+  if (mode == FIN_PRIME_DRIVEN) {
+    plan_next(ctl, sync, dx);
+    return;
+  }
+  kahan_add64(ctl->infil_out, ctl->infil_out_c, inf * area);
+  kahan_add64(ctl->edge_out, ctl->edge_out_c, edge * area);
+  const double dt = ctl->dt;
+  if (mode == FIN_DRIVEN) {
+    // run.py:380-388: the ledger banks the analytic rain volume of the step.
+    if (ctl->run_field) {
+      if (ctl->rain_scale != 0.0f) kahan_add64(ctl->rain_in, ctl->rain_in_c, ctl->rain_field_sum * dt * area);
+    } else {
+      const double r = ctl->rain_scale != 0.0f ? ctl->rain_m_s : 0.0;
+      if (r > 0.0) kahan_add64(ctl->rain_in, ctl->rain_in_c, r * dt * area * n_cells);
+    }
+  }
+  ctl->t = ctl->t + dt;
+  ctl->steps += 1;
+  if (mode == FIN_DRIVEN) plan_next(ctl, sync, dx);
+  else ctl->active = 0;
+}
+
+int ensure_partials(rbs_state *s, int nblk) {
+  if (nblk <= s->part_cap) return RBS_OK;
+  if (s->part.speed_max) {
+    RBS_HIP(hipFree(s->part.speed_max));
+    RBS_HIP(hipFree(s->part.vol));
+    RBS_HIP(hipFree(s->part.infil));
+    RBS_HIP(hipFree(s->part.edge));
+  }
+  RBS_HIP(hipMalloc(&s->part.speed_max, sizeof(float) * nblk));
+  RBS_HIP(hipMalloc(&s->part.vol, sizeof(double) * nblk));
+  RBS_HIP(hipMalloc(&s->part.infil, sizeof(double) * nblk));
+  RBS_HIP(hipMalloc(&s->part.edge, sizeof(double) * nblk));
+  s->part_cap = nblk;
+  return RBS_OK;
+}
+
+static int reduce_blocks(const rbs_state *s, int *rows_per_blk) {
+  // about 4 rows of 4096 per block; at least one row
+  int rpb = (16384 + s->nx - 1) / s->nx;
+  if (rpb < 1) rpb = 1;
+  *rows_per_blk = rpb;
+  return (s->ny + rpb - 1) / rpb;
+}
+
+int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
+  int rpb = 1;
+  const int nblk = reduce_blocks(s, &rpb);
+  if (nblk > s->part_cap) {
+    set_error("partial buffer too small");
+    return RBS_E_STATE;
+  }
+  const float *h = s->h[s->cur];
+  hipLaunchKernelGGL(reduce_speed_kernel, dim3(nblk), dim3(kThreads), 0, s->stream, h, s->hu,
+                     s->hv, s->pitch, s->ny, s->nx, scheme, s->part, rpb);
+  RBS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
+                     nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx,
+                     mode);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
+
+int finalize_launch(rbs_state *s, int mode, int scheme) {
+  const int nblk = scheme == RBS_SCHEME_HLLC ? hllc_num_blocks(s) : li_num_blocks(s);
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
+                     nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx, mode);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
+
+int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale) {
+  hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, s->stream, s->ctl, t, dt, rain,
+                     rain_scale);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
+
+int reduce_partials_needed(const rbs_state *s) {
+  int rpb = 1;
+  return reduce_blocks(s, &rpb);
+}
+
+}  // namespace rbs

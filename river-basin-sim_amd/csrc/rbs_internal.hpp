@@ -1,0 +1,163 @@
+// rbs_internal.hpp -- shared definitions of librbs (gfx950).
+//
+// Device layout: every 2-D field is row-major with a padded pitch (a multiple of
+// 64 elements, so each row starts on a 256-byte boundary).  Cells use columns
+// [0, nx), x-faces [0, nx], rows [0, ny) for cells and x-faces, [0, ny] for
+// y-faces.  All fields of one state share the pitch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/rbs.h"
+
+namespace rbs {
+
+// solver/core/grid.py:44,49 as the float32 kernels see them.
+constexpr float kG = 9.81f;
+constexpr float kHDry = 1.0e-3f;
+constexpr double kGravity = 9.81;
+constexpr double kHDry64 = 1.0e-3;
+// solver/scheduler.py:45
+constexpr double kEpsT = 1e-6;
+// friction.py:117: Python folds 7.0/3.0 in float64; the kernel parameter is float32.
+constexpr float kSevenThirds = (float)(7.0 / 3.0);
+
+constexpr int kThreads = 256;  // block size of every tiled kernel
+
+// ---- error plumbing ---------------------------------------------------------
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define RBS_HIP(call)                                 \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return ::rbs::hip_fail(e_, #call); \
+  } while (0)
+
+// ---- device control block ---------------------------------------------------
+// One per state, in device memory.  The step kernels read their scalars from it,
+// so one captured graph serves every step of a device-driven run.
+struct DevCtl {
+  // clock (scheduler.py:171-246), float64
+  double t;        // simulated time at the start of the next step
+  double dt;       // dt of the next step
+  double t_stop;   // device-driven run target
+  int32_t active;  // 1 while the next step is due
+  int32_t sync_idx;
+  int64_t steps;
+  // per-step forcing (run.py:378-388)
+  float rain;        // uniform rate handed to the step (float32)
+  float rain_scale;  // field multiplier (1 raining, 0 not)
+  float speed_max;   // LI: max h;  HLLC: max wave speed (of the current state)
+  int32_t stale;     // speed_max needs a fresh reduction
+  // ledger, volumes in m^3
+  double volume;
+  double rain_in, rain_in_c;
+  double point_in, point_in_c;
+  double infil_out, infil_out_c;
+  double edge_out, edge_out_c;
+  // device-driven run parameters (copied from rbs_run_params)
+  int32_t run_scheme, run_limit, run_field, n_sync;
+  double alpha, dt_max, rain_m_s, rain_duration, rain_field_sum;
+  int64_t step_limit;  // device-driven run stops after this many steps in total
+};
+
+// Per-block partial results of one step, reduced in fixed order by the finalize
+// kernel (deterministic; no float atomics).
+struct Partials {
+  float *speed_max;   // [nblk]
+  double *vol;        // [nblk] sum f64(h_new)
+  double *infil;      // [nblk] sum infiltration depth removed
+  double *edge;       // [nblk] sum edge drain / banked depth (signed)
+};
+
+struct StageCurve {
+  int32_t n = 0;
+  double *t = nullptr;      // host copies; evaluated per step on the host or device
+  double *level = nullptr;
+  double *d_t = nullptr;    // device copies for device-driven runs
+  double *d_level = nullptr;
+};
+
+struct Inflows {
+  int32_t count = 0;
+  int32_t *d_rows = nullptr, *d_cols = nullptr, *d_off = nullptr;
+  double *d_kt = nullptr, *d_kq = nullptr;
+  float *d_comp = nullptr;      // per-entry Kahan debt (inflow.py:145-151)
+  double *d_vol = nullptr;      // [count] requested volume this step (f64 of f32 depth)
+  int32_t n_knots = 0;
+};
+
+}  // namespace rbs
+
+// The opaque handle.
+struct rbs_state {
+  int32_t ny = 0, nx = 0;
+  double dx = 0.0;
+  int64_t pitch = 0;         // elements per row
+  uint32_t armed = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t bc[4] = {0, 0, 0, 0};
+  int32_t li_variant = 0;
+  // LI ping-pong: index `cur` holds the current state
+  int cur = 0;
+  float *h[2] = {nullptr, nullptr};
+  float *qx[2] = {nullptr, nullptr};
+  float *qy[2] = {nullptr, nullptr};
+  float *z = nullptr, *n = nullptr, *eta = nullptr;
+  // optional
+  float *h_comp = nullptr, *rain = nullptr, *infil = nullptr;
+  float *hu = nullptr, *hv = nullptr, *h1 = nullptr, *hu1 = nullptr, *hv1 = nullptr;
+  float *fh_x = nullptr, *fh_y = nullptr;   // HLLC limited boundary mass fluxes (scratch)
+  float *d50 = nullptr, *qs_int_x = nullptr, *qs_comp_x = nullptr, *qs_int_y = nullptr,
+        *qs_comp_y = nullptr, *z0 = nullptr, *dz_lo = nullptr, *dz_hi = nullptr;
+  double *dz_cum = nullptr, *dz_unapplied = nullptr;
+  // reductions
+  rbs::Partials part{};
+  int32_t part_cap = 0;
+  rbs::DevCtl *ctl = nullptr;        // device
+  rbs::DevCtl *ctl_host = nullptr;   // pinned host mirror for status reads
+  double *d_sync = nullptr;
+  int32_t sync_cap = 0;
+  rbs::StageCurve stage[4];
+  rbs::Inflows inflow;
+  // captured device-driven step graphs, one per starting ping-pong parity
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_key = -1;     // scheme * 2 + limit of the captured graphs
+  int graph_chunk = 16;   // steps per graph (even: a full graph keeps the parity)
+  int speed_scheme = -1;  // scheme whose CFL speed ctl->speed_max holds (-1: stale)
+  uint64_t bytes = 0;
+  double rain_field_sum = 0.0;
+};
+
+namespace rbs {
+
+inline size_t cells_bytes(const rbs_state *s) { return (size_t)s->ny * s->pitch * sizeof(float); }
+inline size_t xf_rows(const rbs_state *s) { return (size_t)s->ny; }
+inline size_t yf_rows(const rbs_state *s) { return (size_t)s->ny + 1; }
+
+// Launch helpers implemented in the kernel translation units.
+int li_launch_step(rbs_state *s, bool device_scalars, double dt, double rain, int limit,
+                   double rain_scale);
+int reduce_speed_launch(rbs_state *s, int scheme, int mode);
+int reduce_partials_needed(const rbs_state *s);
+int finalize_launch(rbs_state *s, int mode, int scheme);
+int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale);
+int hllc_launch_step(rbs_state *s, bool device_scalars, double dt, double rain,
+                     double rain_scale, double t);
+int inflow_launch(rbs_state *s, bool device_scalars, double t, double dt);
+int exner_launch(rbs_state *s, double porosity);
+int li_num_blocks(const rbs_state *s);
+int hllc_num_blocks(const rbs_state *s);
+int li_variant_count();
+int ensure_partials(rbs_state *s, int nblk);
+
+// finalize modes
+enum { FIN_PRIME = 0, FIN_EXPLICIT = 1, FIN_DRIVEN = 2, FIN_PRIME_DRIVEN = 3 };
+
+}  // namespace rbs

@@ -1,0 +1,232 @@
+"""Local-inertial HIP path (librbs through the C-ABI) against the oracle.
+
+Tolerance (north star): after 1000 fixed-dt steps max|dh| <= 1e-5 m and
+max|dq| <= 1e-5 m^2/s.  The fused kernel follows the reference's float32
+operation order with no FMA contraction; the only expected differences are ulps
+of ``powf`` between ROCm's device library and glibc.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def rbs():
+    from rbs import _lib
+    from rbs.core import local_inertial, massbalance, state
+
+    _lib.load()
+    assert _lib.device_count() >= 1
+    return type("R", (), dict(lib=_lib, li=local_inertial, State=state.State,
+                              MassLedger=massbalance.MassLedger))
+
+
+def _pair(rbs, orc, bed, dx, depth, manning, *, threads=8):
+    orc.set_threads(threads)
+    g = rbs.State.from_bed(bed, dx, depth=depth, manning=manning)
+    o = orc.State(bed, dx, depth=depth, manning=manning)
+    return g, o
+
+
+def _cmp(g, o, tol=TOL):
+    dh = np.abs(g.h.numpy() - o.h).max()
+    dqx = np.abs(g.qx.numpy() - o.qx).max()
+    dqy = np.abs(g.qy.numpy() - o.qy).max()
+    assert dh <= tol, f"max|dh| {dh:.3e}"
+    assert dqx <= tol and dqy <= tol, f"max|dq| {max(dqx, dqy):.3e}"
+    return dh, max(dqx, dqy)
+
+
+def test_cfg1_dambreak_1000_steps_parity_mass_and_stoker(rbs, orc):
+    """512x512 dam break (h=1 for x<256, 0.1 else), n=0.01, closed, fixed dt."""
+    ny = nx = 512
+    depth = np.where(np.arange(nx)[None, :] < 256, 1.0, 0.1) * np.ones((ny, 1))
+    g, o = _pair(rbs, orc, np.zeros((ny, nx)), 1.0, depth, 0.01)
+    dt = 0.7 * 1.0 / math.sqrt(9.81 * 1.0)  # compute_dt at t = 0 (h_max = 1)
+    assert rbs.li.compute_dt(g, 0.7, 30.0) == dt == orc.li_compute_dt(o, 0.7, 30.0)
+    led = rbs.MassLedger.from_state(g)
+    for k in range(250):
+        rbs.li.step(g, dt)
+    orc.li_run_fixed(o, dt, 250)
+    _cmp(g, o)
+    # Stoker along the centreline row at t = 250 dt (both waves still interior)
+    x = (np.arange(nx) + 0.5 - 256.0) * 1.0
+    t = 250 * dt
+    h_ref, _ = orc.stoker(x, t, 1.0, 0.1)
+    h = g.h.numpy()[256].astype(np.float64)
+    inter = np.abs(x) < 0.9 * 256
+    nrmse = math.sqrt(np.mean((h[inter] - h_ref[inter]) ** 2)) / 0.9
+    lvl = 0.5 * (0.1 + orc.stoker_star_depth(1.0, 0.1))
+    front = x[h > lvl].max()
+    front_err = abs(front - orc.stoker_front_position(t, 1.0, 0.1)) / orc.stoker_front_position(
+        t, 1.0, 0.1)
+    assert nrmse < 0.10 and front_err < 0.15  # validation/test_dam_break.py:49
+    for k in range(750):
+        rbs.li.step(g, dt)
+    orc.li_run_fixed(o, dt, 750)
+    dh, dq = _cmp(g, o)
+    rec = led.record(g, 1000 * dt)
+    assert rec.rel_error < 1e-7, f"mass closure {rec.rel_error:.2e}"
+    print(f"\n[cfg1 parity] max|dh|={dh:.2e} max|dq|={dq:.2e} mass={rec.rel_error:.2e} "
+          f"stoker nRMSE={nrmse:.4f} front={front_err:.4f}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 37), (41, 1), (2, 2), (67, 133), (130, 65)])
+def test_odd_shapes_match(rbs, orc, shape):
+    """Tiles that overhang the grid, single rows/columns, 1x1."""
+    rng = np.random.default_rng(sum(shape))
+    ny, nx = shape
+    z = rng.uniform(0, 1, shape)
+    h = rng.uniform(0.0, 1.0, shape)
+    n = rng.uniform(0.02, 0.08, shape)
+    g, o = _pair(rbs, orc, z, 2.0, h, n)
+    for _ in range(40):
+        dt = rbs.li.compute_dt(g, 0.7, 1.0)
+        assert dt == orc.li_compute_dt(o, 0.7, 1.0)
+        rbs.li.step(g, dt)
+        orc.li_step(o, dt)
+    _cmp(g, o, tol=1e-6)
+
+
+def test_sources_sinks_open_edges_parity(rbs, orc):
+    """Rain field (Kahan), infiltration, all edges open, on a sloped random bed."""
+    rng = np.random.default_rng(5)
+    ny, nx = 96, 160
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.02 * xx + 0.01 * yy + 0.3 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    rain = rng.uniform(0.5, 2.0, (ny, nx)).astype(np.float32) * 1e-5
+    infil = rng.uniform(0, 10, (ny, nx)).astype(np.float32) / 1000 / 3600
+    n = rng.uniform(0.02, 0.1, (ny, nx))
+    g, o = _pair(rbs, orc, z, 10.0, 0.0, n)
+    for s in (g, o):
+        s.set_rain_field(rain)
+        s.set_infiltration(infil)
+        s.set_open_boundaries({e: "open" for e in ("north", "south", "east", "west")})
+        s.arm_source_compensation()
+    led_g = rbs.MassLedger.from_state(g)
+    led_o = orc.MassLedger.from_state(o)
+    rain_sum = float(rain.astype(np.float64).sum())
+    t = 0.0
+    for k in range(600):
+        scale = 1.0 if k < 400 else 0.0
+        dt = rbs.li.compute_dt(g, 0.7, 5.0)
+        assert dt == orc.li_compute_dt(o, 0.7, 5.0)
+        rbs.li.step(g, dt, rain_scale=scale)
+        orc.li_step(o, dt, rain_scale=scale)
+        if scale:
+            led_g.add_inflow(rain_sum * dt * 100.0)
+            led_o.add_inflow(rain_sum * dt * 100.0)
+        t += dt
+    _cmp(g, o)
+    np.testing.assert_allclose(g.h_comp.numpy(), o.h_comp, atol=1e-7)
+    rg, ro = led_g.record(g, t), led_o.record(o, t)
+    assert rg.outflow_cum == pytest.approx(ro.outflow_cum, rel=1e-9)
+    assert rg.rel_error < 1e-6 and ro.rel_error < 1e-6
+
+
+def test_kahan_sub_ulp_and_canary(rbs):
+    """test_sources.py:45-86 through the HIP path."""
+    rain, steps = 4.0e-8, 2000
+    plain = rbs.State.from_bed(np.zeros((24, 24)), 100.0, depth=1.0, manning=0.03)
+    comp = rbs.State.from_bed(np.zeros((24, 24)), 100.0, depth=1.0, manning=0.03)
+    comp.arm_source_compensation()
+    for _ in range(steps):
+        rbs.li.step(plain, 1.0, rain=rain, limit=False)
+        rbs.li.step(comp, 1.0, rain=rain, limit=False)
+    got_plain = plain.h.numpy().astype(np.float64).mean() - 1.0
+    got_comp = comp.h.numpy().astype(np.float64).mean() - 1.0
+    assert got_plain == pytest.approx(0.0, abs=1e-9)
+    assert got_comp == pytest.approx(rain * steps, rel=2e-3)
+    assert np.any(comp.h_comp.numpy() != 0.0)
+
+
+def test_rain_field_is_read_per_cell(rbs):
+    """test_sources.py:179-199."""
+    st = rbs.State.from_bed(np.zeros((24, 24)), 100.0, depth=1.0, manning=0.03)
+    field = np.zeros((24, 24), np.float32)
+    field[:, :12] = 4.0e-8
+    st.set_rain_field(field)
+    st.arm_source_compensation()
+    rbs.li.step(st, 1.0, rain_scale=1.0)
+    comp = st.h_comp.numpy()
+    assert np.all(comp[:, :12] != 0.0) and np.all(comp[:, 12:] == 0.0)
+
+
+def test_rest_flux_sign_and_known_dt(rbs):
+    """test_local_inertial.py:21-43, 132-139."""
+    st = rbs.State.from_bed(np.zeros((8, 8)), 10.0, depth=1.5, manning=0.03)
+    for _ in range(20):
+        rbs.li.step(st, 1.0)
+    assert np.all(st.h.numpy() == np.float32(1.5))
+    st2 = rbs.State.from_bed(np.zeros((1, 2)), 1.0, depth=np.array([[2.0, 1.0]]), manning=0.03)
+    rbs.li.step(st2, 0.01)
+    qx = st2.qx.numpy()
+    assert qx[0, 0] == 0.0 and qx[0, 2] == 0.0 and qx[0, 1] > 0.0
+    st3 = rbs.State.from_bed(np.zeros((4, 4)), 10.0, depth=4.0)
+    assert rbs.li.compute_dt(st3, 0.7, 30.0) == 0.7 * 10.0 / math.sqrt(9.81 * 4.0)
+
+
+def test_determinism_and_variants_are_bitwise(rbs):
+    """Two runs match bit for bit; every tile variant gives the same bits."""
+    rng = np.random.default_rng(9)
+    z = rng.uniform(0, 2, (200, 300)).astype(np.float32)
+    h = rng.uniform(0, 1, (200, 300)).astype(np.float32)
+    count = np.zeros(1, np.int32)
+    import ctypes
+
+    rbs.lib.call("rbs_li_variant_count", count.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    outs = []
+    for v in [0, 0] + list(range(1, int(count[0]))):
+        st = rbs.State.from_bed(z, 5.0, depth=h, manning=0.04)
+        rbs.lib.call("rbs_set_li_variant", st._handle, v)
+        for _ in range(30):
+            rbs.li.step(st, rbs.li.compute_dt(st, 0.7, 2.0))
+        outs.append((st.h.numpy(), st.qx.numpy(), st.qy.numpy()))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_device_driven_run_matches_host_loop(rbs):
+    """rbs_run_until == the reference's host loop (compute_dt, fill_span, step), bitwise."""
+    from rbs.driver import DeviceRun
+    from rbs.scheduler import MultiRateScheduler
+
+    rng = np.random.default_rng(2)
+    z = rng.uniform(0, 0.5, (128, 160)).astype(np.float32)
+    rain_m_s = 20.0 / 1000 / 3600
+    sched = MultiRateScheduler(end_time=600.0, output_every=150.0, events=[400.0])
+
+    host = rbs.State.from_bed(z, 10.0, depth=0.05, manning=0.03)
+    host.arm_source_compensation()
+    dev = rbs.State.from_bed(z, 10.0, depth=0.05, manning=0.03)
+    dev.arm_source_compensation()
+    led_h = rbs.MassLedger.from_state(host)
+    led_d = rbs.MassLedger.from_state(dev)
+    n_host = 0
+    for tick in sched.ticks(lambda: rbs.li.compute_dt(host, 0.7, 30.0)):
+        rain = rain_m_s if tick.t0 < 400.0 - 1e-6 else 0.0
+        rbs.li.step(host, tick.dt, rain=rain, t=tick.t0)
+        if rain > 0:
+            led_h.add_rain_step(rain, tick.dt, host.grid.n_cells)
+        n_host += 1
+    run = DeviceRun(dev, "local_inertial", sched, alpha=0.7, dt_max=30.0, rain_m_s=rain_m_s,
+                    rain_duration=400.0)
+    n_dev = 0
+    for seg in sched.segments():
+        n_dev += run.run_until(seg.t_stop)
+    assert n_dev == n_host
+    assert np.array_equal(host.h.numpy(), dev.h.numpy())
+    assert np.array_equal(host.qx.numpy(), dev.qx.numpy())
+    rh, rd = led_h.record(host, 600.0), led_d.record(dev, 600.0)
+    assert rd.inflow_cum == pytest.approx(rh.inflow_cum, rel=1e-12)
+    assert dev.ledger().t == pytest.approx(600.0, abs=1e-9)
+    assert rd.rel_error < 1e-6
