@@ -50,12 +50,31 @@ __global__ __launch_bounds__(kThreads) void reduce_speed_kernel(const float *h, 
 }
 
 // Host-driven step: write the scalars the step kernels read.
-__global__ void set_scalars_kernel(DevCtl *ctl, double t, double dt, float rain, float scale) {
+__global__ void set_scalars_kernel(DevCtl *ctl, double t, double dt, float rain, float scale,
+                                   float s0, float s1, float s2, float s3) {
   ctl->t = t;
   ctl->dt = dt;
   ctl->rain = rain;
   ctl->rain_scale = scale;
+  ctl->stage[0] = s0;
+  ctl->stage[1] = s1;
+  ctl->stage[2] = s2;
+  ctl->stage[3] = s3;
   ctl->active = 1;
+}
+
+// boundaries.py:49-70 stage_at: piecewise linear in float64, held outside its range.
+__device__ double stage_at_dev(const double *t, const double *lv, int n, double x) {
+  if (x <= t[0]) return lv[0];
+  if (x >= t[n - 1]) return lv[n - 1];
+  for (int k = 0; k + 1 < n; ++k) {
+    const double t0 = t[k], t1 = t[k + 1];
+    if (t0 <= x && x <= t1) {
+      if (t1 == t0) return lv[k + 1];
+      return lv[k] + (lv[k + 1] - lv[k]) * (x - t0) / (t1 - t0);
+    }
+  }
+  return lv[n - 1];
 }
 
 // Next step of a device-driven run (scheduler.py:171-229 + run.py:378-388).
@@ -97,6 +116,11 @@ __device__ void plan_next(DevCtl *ctl, const double *sync, double dx) {
   const bool raining = t < ctl->rain_duration - kEpsT;  // run.py:378
   ctl->rain = ctl->run_field ? 0.0f : (raining ? (float)ctl->rain_m_s : 0.0f);
   ctl->rain_scale = raining ? 1.0f : 0.0f;
+  // hllc.py:1308-1312: stages at the step midpoint, narrowed to float32
+  for (int e = 0; e < 4; ++e)
+    if (ctl->stage_n[e] > 0)
+      ctl->stage[e] = (float)stage_at_dev(ctl->stage_t[e], ctl->stage_lv[e], ctl->stage_n[e],
+                                          t + 0.5 * dt);
   ctl->active = 1;
 }
 
@@ -195,9 +219,12 @@ int finalize_launch(rbs_state *s, int mode, int scheme) {
   return RBS_OK;
 }
 
-int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale) {
+int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale,
+                       const float stage[4]) {
+  const float z4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const float *st = stage ? stage : z4;
   hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, s->stream, s->ctl, t, dt, rain,
-                     rain_scale);
+                     rain_scale, st[0], st[1], st[2], st[3]);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

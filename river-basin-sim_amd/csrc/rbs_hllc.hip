@@ -1,0 +1,526 @@
+// rbs_hllc.hip -- well-balanced HLLC finite-volume step for gfx950.
+//
+// hllc.step (hllc.py:1272-1373) is SSP-RK2 over the spatial operator _eval_L
+// (hllc.py:1152-1269).  Each RK stage is one fused kernel over BY x BX tiles:
+//   primitives u, v, eta of the stage state           (_primitives :262-282)
+//   MUSCL-minmod + dry drop + Audusse + HLLC per face (_flux_x/_flux_y :285-424)
+//   per-edge ghosts: fixed_stage, then closed walls    (_apply_boundaries :993-1109)
+//   donor beta on the mass flux, ghost inflow unscaled (_mass_beta/_limit_* :700-773)
+//   boundary banking, 0.5*dt/dx per stage            (_bank_* :638-677, :1115-1149)
+//   flux divergence + centred bed-slope source        (_accumulate :776-847)
+//   stage 1: Euler predictor U1                       (_rk_stage1 :850-878)
+//   stage 2: Heun corrector in place, then friction, sources and the CFL
+//            wave-speed partials                       (:881-990, :1338-1373)
+// A tile stages h, eta, z, u, v three cells out in LDS (the 4-point MUSCL stencil
+// of the ring-1 cells' faces), computes every face of the ring-1 cells (their
+// mass fluxes feed the donor beta the tile's own faces need), then works on its
+// own cells.  Stage 1 writes U1 to separate buffers; stage 2 reads U1 with a halo
+// and U^n only at its own cell, so it may overwrite U^n in place.
+#include "rbs_device.hpp"
+
+namespace rbs {
+
+struct HlArgs {
+  const float *h_in, *hu_in, *hv_in;  // stage input (U^n or U1)
+  float *h_out, *hu_out, *hv_out;     // stage output (U1, or U^n in place)
+  const float *hn, *hun, *hvn;        // stage 2: U^n
+  const float *z, *n;
+  float *h_comp;
+  const float *rain, *infil;
+  const DevCtl *ctl;
+  Partials part;
+  double dx64;
+  int64_t pitch;
+  int32_t ny, nx, tiles_x, n_tiles;
+  int32_t bc[4];
+  int32_t flags;  // HL_* below
+};
+
+enum : int { HL_COMP = 1, HL_FIELD = 2, HL_INFIL = 4, HL_LOSS = 8 };
+
+// hllc.py:158-165 _minmod
+__device__ __forceinline__ float minmod(float a, float b) {
+  if (a * b <= 0.0f) return 0.0f;
+  if (fabsf(a) < fabsf(b)) return a;
+  return b;
+}
+
+// hllc.py:174-188 _dryfactor
+__device__ __forceinline__ float dryfactor(float h0, float hm, float hp) {
+  return (h0 <= kHDry || hm <= kHDry || hp <= kHDry) ? 0.0f : 1.0f;
+}
+
+// hllc.py:191-256 _hllc: (mass, normal momentum, transverse momentum) flux.
+__device__ __forceinline__ void hllc_flux(float hl, float ul, float vtl, float hr, float ur,
+                                          float vtr, float &fh, float &fm, float &ft) {
+  const float g = kG, dry = kHDry;
+  const float cl = sqrtf(g * fmx(hl, 0.0f));
+  const float cr = sqrtf(g * fmx(hr, 0.0f));
+  const bool dry_l = hl <= dry, dry_r = hr <= dry;
+  if (dry_l && dry_r) {
+    fh = fm = ft = 0.0f;
+    return;
+  }
+  float sl, sr;
+  if (dry_l) {
+    sl = ur - 2.0f * cr;
+    sr = ur + cr;
+  } else if (dry_r) {
+    sl = ul - cl;
+    sr = ul + 2.0f * cl;
+  } else {
+    const float hs = 0.5f * (cl + cr) + 0.25f * (ul - ur);
+    const float h_star = fmx(hs * hs / g, 0.0f);
+    const float c_star = sqrtf(g * h_star);
+    const float u_star = 0.5f * (ul + ur) + cl - cr;
+    sl = fmn(ul - cl, u_star - c_star);
+    sr = fmx(ur + cr, u_star + c_star);
+  }
+  const float fh_l = hl * ul;
+  const float fmn_l = hl * ul * ul + 0.5f * g * hl * hl;
+  const float fh_r = hr * ur;
+  const float fmn_r = hr * ur * ur + 0.5f * g * hr * hr;
+  if (sl >= 0.0f) {
+    fh = fh_l;
+    fm = fmn_l;
+  } else if (sr <= 0.0f) {
+    fh = fh_r;
+    fm = fmn_r;
+  } else {
+    const float inv = 1.0f / (sr - sl);
+    fh = (sr * fh_l - sl * fh_r + sl * sr * (hr - hl)) * inv;
+    fm = (sr * fmn_l - sl * fmn_r + sl * sr * (hr * ur - hl * ul)) * inv;
+  }
+  const float den_s = hr * (ur - sr) - hl * (ul - sl);
+  float s_star = 0.5f * (ul + ur);
+  if (fabsf(den_s) > 1.0e-12f) s_star = (sl * hr * (ur - sr) - sr * hl * (ul - sl)) / den_s;
+  ft = fh * (s_star >= 0.0f ? vtl : vtr);
+}
+
+// One face of _flux_x / _flux_y: 4-point stencil p0..p3 along the face normal
+// (cells f-2, f-1 | f, f+1); un is the normal velocity, ut the transverse one.
+__device__ __forceinline__ void muscl_face(const float e[4], const float zz[4], const float un[4],
+                                           const float ut[4], const float hh[4], float &fh,
+                                           float &fl, float &fr, float &ft) {
+  const float g = kG;
+  const float wl = dryfactor(hh[1], hh[0], hh[2]);
+  const float wr = dryfactor(hh[2], hh[1], hh[3]);
+  const float eta_l = e[1] + 0.5f * wl * minmod(e[1] - e[0], e[2] - e[1]);
+  const float eta_r = e[2] - 0.5f * wr * minmod(e[2] - e[1], e[3] - e[2]);
+  const float zf_l = zz[1] + 0.5f * wl * minmod(zz[1] - zz[0], zz[2] - zz[1]);
+  const float zf_r = zz[2] - 0.5f * wr * minmod(zz[2] - zz[1], zz[3] - zz[2]);
+  const float un_l = un[1] + 0.5f * wl * minmod(un[1] - un[0], un[2] - un[1]);
+  const float un_r = un[2] - 0.5f * wr * minmod(un[2] - un[1], un[3] - un[2]);
+  const float ut_l = ut[1] + 0.5f * wl * minmod(ut[1] - ut[0], ut[2] - ut[1]);
+  const float ut_r = ut[2] - 0.5f * wr * minmod(ut[2] - ut[1], ut[3] - ut[2]);
+  const float h_l = fmx(eta_l - zf_l, 0.0f);
+  const float h_r = fmx(eta_r - zf_r, 0.0f);
+  const float z_star = fmx(zf_l, zf_r);
+  const float hl_star = fmx(eta_l - z_star, 0.0f);
+  const float hr_star = fmx(eta_r - z_star, 0.0f);
+  float f0, f1, f2;
+  hllc_flux(hl_star, un_l, ut_l, hr_star, un_r, ut_r, f0, f1, f2);
+  fh = f0;
+  fl = f1 + 0.5f * g * (h_l * h_l - hl_star * hl_star);
+  fr = f1 + 0.5f * g * (h_r * h_r - hr_star * hr_star);
+  ft = f2;
+}
+
+template <int BX, int BY, int STAGE>
+__global__ __launch_bounds__(kThreads) void hllc_stage_kernel(HlArgs a) {
+  const DevCtl *ctl = a.ctl;
+  if (!ctl->active) return;
+
+  constexpr int EW = BX + 6, EH = BY + 6;  // cells three out
+  constexpr int RW = BX + 2, RH = BY + 2;  // ring-1 cells
+  constexpr int XW = BX + 3;               // x-faces of ring-1 cells (per row)
+  constexpr int YH = BY + 3;               // y-face rows of ring-1 cells
+  __shared__ float s_h[EH][EW], s_e[EH][EW], s_z[EH][EW], s_u[EH][EW], s_v[EH][EW];
+  __shared__ float s_fxh[RH][XW], s_fxl[RH][XW], s_fxr[RH][XW], s_fxt[RH][XW];
+  __shared__ float s_fyh[YH][RW], s_fyl[YH][RW], s_fyr[YH][RW], s_fyt[YH][RW];
+  __shared__ float s_beta[RH][RW];
+  __shared__ double s_red[4 * (kThreads / 64)];
+
+  const int tile = xcd_tile(blockIdx.x, a.n_tiles);
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int y0 = ty * BY, x0 = tx * BX;
+  const int ny = a.ny, nx = a.nx;
+  const int64_t P = a.pitch;
+  const int tid = threadIdx.x;
+  const double dt64 = ctl->dt;
+  const float dt = (float)dt64;
+  const float dx = (float)a.dx64;
+  const float dtdx = dt / dx;
+
+  // ---- stage the halo'd state; primitives (_primitives) ------------------------
+#pragma unroll
+  for (int k0 = 0; k0 < EH * EW; k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < EH * EW) {
+      const int lr = k / EW, lc = k - (k / EW) * EW;
+      const int gi = clampi(y0 + lr - 3, 0, ny - 1), gj = clampi(x0 + lc - 3, 0, nx - 1);
+      const int64_t o = (int64_t)gi * P + gj;
+      const float hh = a.h_in[o];
+      const float zv = a.z[o];
+      float uu = 0.0f, vv = 0.0f;
+      if (hh > kHDry) {
+        uu = a.hu_in[o] / hh;
+        vv = a.hv_in[o] / hh;
+      }
+      s_h[lr][lc] = hh;
+      s_e[lr][lc] = hh + zv;
+      s_z[lr][lc] = zv;
+      s_u[lr][lc] = uu;
+      s_v[lr][lc] = vv;
+    }
+  }
+  __syncthreads();
+
+  const int bw = a.bc[RBS_EDGE_WEST], be = a.bc[RBS_EDGE_EAST];
+  const int bn = a.bc[RBS_EDGE_NORTH], bs = a.bc[RBS_EDGE_SOUTH];
+  // ---- x-faces of the ring-1 cells ----------------------------------------------
+#pragma unroll
+  for (int k0 = 0; k0 < RH * XW; k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < RH * XW) {
+      const int lr = k / XW, lf = k - (k / XW) * XW;
+      const int r = y0 + lr - 1, f = x0 + lf - 1;
+      float fh = 0.0f, fl = 0.0f, fr = 0.0f, ft = 0.0f;
+      if (r >= 0 && r < ny && f >= 0 && f <= nx) {
+        const int er = lr + 2;  // R3 row of r; cells f-2..f+1 are R3 cols lf..lf+3
+        if (f == 0 && bw == RBS_BC_FIXED_STAGE) {  // _stage_x_west
+          const float h_g = fmx(ctl->stage[RBS_EDGE_WEST] - s_z[er][lf + 2], 0.0f);
+          const float u = s_u[er][lf + 2], v = s_v[er][lf + 2];
+          hllc_flux(h_g, u, v, s_h[er][lf + 2], u, v, fh, fr, ft);
+          fl = fr;
+        } else if (f == nx && be == RBS_BC_FIXED_STAGE) {  // _stage_x_east
+          const float u = s_u[er][lf + 1], v = s_v[er][lf + 1];
+          const float h_g = fmx(ctl->stage[RBS_EDGE_EAST] - s_z[er][lf + 1], 0.0f);
+          hllc_flux(s_h[er][lf + 1], u, v, h_g, u, v, fh, fl, ft);
+          fr = fl;
+        } else if (f == 0 && bw == RBS_BC_CLOSED) {  // _wall_x_west
+          const float h0 = s_h[er][lf + 2], u = s_u[er][lf + 2], v = s_v[er][lf + 2];
+          hllc_flux(h0, -u, v, h0, u, v, fh, fr, ft);
+          fl = fr;
+        } else if (f == nx && be == RBS_BC_CLOSED) {  // _wall_x_east
+          const float h0 = s_h[er][lf + 1], u = s_u[er][lf + 1], v = s_v[er][lf + 1];
+          hllc_flux(h0, u, v, h0, -u, v, fh, fl, ft);
+          fr = fl;
+        } else {  // interior, or open edge (transmissive: clamped neighbours)
+          float e[4], zz[4], un[4], ut[4], hh[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            e[m] = s_e[er][lf + m];
+            zz[m] = s_z[er][lf + m];
+            un[m] = s_u[er][lf + m];
+            ut[m] = s_v[er][lf + m];
+            hh[m] = s_h[er][lf + m];
+          }
+          muscl_face(e, zz, un, ut, hh, fh, fl, fr, ft);
+        }
+      }
+      s_fxh[lr][lf] = fh;
+      s_fxl[lr][lf] = fl;
+      s_fxr[lr][lf] = fr;
+      s_fxt[lr][lf] = ft;
+    }
+  }
+  // ---- y-faces of the ring-1 cells (normal v, transverse u) -----------------------
+#pragma unroll
+  for (int k0 = 0; k0 < YH * RW; k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < YH * RW) {
+      const int lf = k / RW, lc = k - (k / RW) * RW;
+      const int f = y0 + lf - 1, c = x0 + lc - 1;
+      float fh = 0.0f, fl = 0.0f, fr = 0.0f, ft = 0.0f;
+      if (c >= 0 && c < nx && f >= 0 && f <= ny) {
+        const int ec = lc + 2;  // R3 col of c; rows f-2..f+1 are R3 rows lf..lf+3
+        if (f == 0 && bn == RBS_BC_FIXED_STAGE) {  // _stage_y_north
+          const float h_g = fmx(ctl->stage[RBS_EDGE_NORTH] - s_z[lf + 2][ec], 0.0f);
+          const float u = s_u[lf + 2][ec], v = s_v[lf + 2][ec];
+          hllc_flux(h_g, v, u, s_h[lf + 2][ec], v, u, fh, fr, ft);
+          fl = fr;
+        } else if (f == ny && bs == RBS_BC_FIXED_STAGE) {  // _stage_y_south
+          const float u = s_u[lf + 1][ec], v = s_v[lf + 1][ec];
+          const float h_g = fmx(ctl->stage[RBS_EDGE_SOUTH] - s_z[lf + 1][ec], 0.0f);
+          hllc_flux(s_h[lf + 1][ec], v, u, h_g, v, u, fh, fl, ft);
+          fr = fl;
+        } else if (f == 0 && bn == RBS_BC_CLOSED) {  // _wall_y_north
+          const float h0 = s_h[lf + 2][ec], u = s_u[lf + 2][ec], v = s_v[lf + 2][ec];
+          hllc_flux(h0, -v, u, h0, v, u, fh, fr, ft);
+          fl = fr;
+        } else if (f == ny && bs == RBS_BC_CLOSED) {  // _wall_y_south
+          const float h0 = s_h[lf + 1][ec], u = s_u[lf + 1][ec], v = s_v[lf + 1][ec];
+          hllc_flux(h0, v, u, h0, -v, u, fh, fl, ft);
+          fr = fl;
+        } else {
+          float e[4], zz[4], un[4], ut[4], hh[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            e[m] = s_e[lf + m][ec];
+            zz[m] = s_z[lf + m][ec];
+            un[m] = s_v[lf + m][ec];
+            ut[m] = s_u[lf + m][ec];
+            hh[m] = s_h[lf + m][ec];
+          }
+          muscl_face(e, zz, un, ut, hh, fh, fl, fr, ft);
+        }
+      }
+      s_fyh[lf][lc] = fh;
+      s_fyl[lf][lc] = fl;
+      s_fyr[lf][lc] = fr;
+      s_fyt[lf][lc] = ft;
+    }
+  }
+  __syncthreads();
+
+  // ---- _mass_beta of the ring-1 cells ----------------------------------------------
+#pragma unroll
+  for (int k0 = 0; k0 < RH * RW; k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < RH * RW) {
+      const int lr = k / RW, lc = k - (k / RW) * RW;
+      float q_out = 0.0f;
+      const float fl = s_fxh[lr][lc];
+      if (fl < 0.0f) q_out = q_out - fl;
+      const float fr = s_fxh[lr][lc + 1];
+      if (fr > 0.0f) q_out = q_out + fr;
+      const float ft = s_fyh[lr][lc];
+      if (ft < 0.0f) q_out = q_out - ft;
+      const float fb = s_fyh[lr + 1][lc];
+      if (fb > 0.0f) q_out = q_out + fb;
+      const float od = dtdx * q_out;
+      s_beta[lr][lc] = od > 0.0f ? fclamp(s_h[lr + 2][lc + 2] / od, 0.0f, 1.0f) : 1.0f;
+    }
+  }
+  __syncthreads();
+
+  // ---- _limit_fx / _limit_fy on the tile's faces, banking of the edge faces ---------
+  double bank = 0.0;
+  const double wt = 0.5 * dt64 / a.dx64;  // hllc.py:1320 (float64)
+  const bool loss = a.flags & HL_LOSS;
+#pragma unroll
+  for (int k0 = 0; k0 < BY * (BX + 1); k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < BY * (BX + 1)) {
+      const int lr = k / (BX + 1), lfo = k - (k / (BX + 1)) * (BX + 1);
+      const int r = y0 + lr, f = x0 + lfo;
+      if (r < ny && f <= nx) {
+        const int lf = lfo + 1;  // ring-face index
+        float q = s_fxh[lr + 1][lf];
+        if (q > 0.0f) {
+          if (f >= 1) q = q * s_beta[lr + 1][lf - 1];
+        } else if (q < 0.0f) {
+          if (f <= nx - 1) q = q * s_beta[lr + 1][lf];
+        }
+        s_fxh[lr + 1][lf] = q;
+        if (loss) {
+          if (f == 0 && (bw == RBS_BC_OPEN || bw == RBS_BC_FIXED_STAGE)) bank -= wt * (double)q;
+          if (f == nx && (be == RBS_BC_OPEN || be == RBS_BC_FIXED_STAGE)) bank += wt * (double)q;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k0 = 0; k0 < (BY + 1) * BX; k0 += kThreads) {
+    const int k = k0 + tid;
+    if (k < (BY + 1) * BX) {
+      const int lfo = k / BX, lc = k - (k / BX) * BX;
+      const int f = y0 + lfo, c = x0 + lc;
+      if (f <= ny && c < nx) {
+        const int lf = lfo + 1;
+        float q = s_fyh[lf][lc + 1];
+        if (q > 0.0f) {
+          if (f >= 1) q = q * s_beta[lf - 1][lc + 1];
+        } else if (q < 0.0f) {
+          if (f <= ny - 1) q = q * s_beta[lf][lc + 1];
+        }
+        s_fyh[lf][lc + 1] = q;
+        if (loss) {
+          if (f == 0 && (bn == RBS_BC_OPEN || bn == RBS_BC_FIXED_STAGE)) bank -= wt * (double)q;
+          if (f == ny && (bs == RBS_BC_OPEN || bs == RBS_BC_FIXED_STAGE)) bank += wt * (double)q;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- _accumulate + RK stage (+ friction, sources on stage 2) -----------------------
+  const float inv_dx = 1.0f / dx;
+  const float g = kG;
+  float m_s = 0.0f;
+  double s_vol = 0.0, s_inf = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < BY * BX; k0 += kThreads) {
+    const int k = k0 + tid;
+    const int lr = k / BX, lc = k - (k / BX) * BX;
+    const int r = y0 + lr, c = x0 + lc;
+    if (k < BY * BX && r < ny && c < nx) {
+      const int er = lr + 3, ec = lc + 3;
+      const float dh = -((s_fxh[lr + 1][lc + 2] - s_fxh[lr + 1][lc + 1]) +
+                         (s_fyh[lr + 2][lc + 1] - s_fyh[lr + 1][lc + 1])) * inv_dx;
+      const float hg = s_h[er][ec];
+      const float wx = dryfactor(hg, s_h[er][ec - 1], s_h[er][ec + 1]);
+      const float wy = dryfactor(hg, s_h[er - 1][ec], s_h[er + 1][ec]);
+      const float zc = s_z[er][ec], ec_ = s_e[er][ec];
+      const float sz_x = wx * minmod(zc - s_z[er][ec - 1], s_z[er][ec + 1] - zc);
+      const float se_x = wx * minmod(ec_ - s_e[er][ec - 1], s_e[er][ec + 1] - ec_);
+      const float hp_x = fmx((ec_ + 0.5f * se_x) - (zc + 0.5f * sz_x), 0.0f);
+      const float hm_x = fmx((ec_ - 0.5f * se_x) - (zc - 0.5f * sz_x), 0.0f);
+      const float src_x = -g * 0.5f * (hp_x + hm_x) * sz_x;
+      const float dhu = (-((s_fxl[lr + 1][lc + 2] - s_fxr[lr + 1][lc + 1]) +
+                           (s_fyt[lr + 2][lc + 1] - s_fyt[lr + 1][lc + 1])) + src_x) * inv_dx;
+      const float sz_y = wy * minmod(zc - s_z[er - 1][ec], s_z[er + 1][ec] - zc);
+      const float se_y = wy * minmod(ec_ - s_e[er - 1][ec], s_e[er + 1][ec] - ec_);
+      const float hp_y = fmx((ec_ + 0.5f * se_y) - (zc + 0.5f * sz_y), 0.0f);
+      const float hm_y = fmx((ec_ - 0.5f * se_y) - (zc - 0.5f * sz_y), 0.0f);
+      const float src_y = -g * 0.5f * (hp_y + hm_y) * sz_y;
+      const float dhv = (-((s_fyl[lr + 2][lc + 1] - s_fyr[lr + 1][lc + 1]) +
+                           (s_fxt[lr + 1][lc + 2] - s_fxt[lr + 1][lc + 1])) + src_y) * inv_dx;
+      const int64_t o = (int64_t)r * P + c;
+      if (STAGE == 1) {  // _rk_stage1
+        const float hh = fmx(hg + dt * dh, 0.0f);
+        float hu1 = 0.0f, hv1 = 0.0f;
+        if (hh > kHDry) {
+          hu1 = a.hu_in[o] + dt * dhu;
+          hv1 = a.hv_in[o] + dt * dhv;
+        }
+        a.h_out[o] = hh;
+        a.hu_out[o] = hu1;
+        a.hv_out[o] = hv1;
+      } else {  // _rk_stage2, in place on U^n
+        float hh = fmx(0.5f * (a.hn[o] + hg + dt * dh), 0.0f);
+        float hu = 0.0f, hv = 0.0f;
+        if (hh > kHDry) {
+          hu = 0.5f * (a.hun[o] + a.hu_in[o] + dt * dhu);
+          hv = 0.5f * (a.hvn[o] + a.hv_in[o] + dt * dhv);
+        }
+        // _friction (:912-939)
+        if (hh <= kHDry) {
+          hu = 0.0f;
+          hv = 0.0f;
+        } else {
+          const float mom = sqrtf(hu * hu + hv * hv);
+          const float den = manning_den(mom, hh, a.n[o], g, dt);
+          hu = hu / den;
+          hv = hv / den;
+        }
+        // sources (:1350-1373)
+        const float rain = ctl->rain;
+        if ((a.flags & HL_COMP) && !(a.flags & HL_FIELD)) {
+          float cp = a.h_comp[o];
+          kahan_add(hh, cp, rain * dt);
+          a.h_comp[o] = cp;
+        } else if (rain != 0.0f) {
+          hh = hh + rain * dt;
+        }
+        if (a.flags & HL_FIELD) {
+          const float inc = a.rain[o] * ctl->rain_scale * dt;
+          if (a.flags & HL_COMP) {
+            float cp = a.h_comp[o];
+            kahan_add(hh, cp, inc);
+            a.h_comp[o] = cp;
+          } else {
+            hh = hh + inc;
+          }
+        }
+        if (a.flags & HL_INFIL) {
+          const float avail = hh;
+          float inf = a.infil[o] * dt;
+          if (inf > avail) inf = avail;
+          if (inf < 0.0f) inf = 0.0f;
+          hh = avail - inf;
+          s_inf += (double)avail - (double)hh;
+        }
+        a.h_out[o] = hh;
+        a.hu_out[o] = hu;
+        a.hv_out[o] = hv;
+        // _reduce_wavespeed (:948-966) of the new state
+        const float cc = sqrtf(g * fmx(hh, 0.0f));
+        float sp = cc;
+        if (hh > kHDry) sp = fmx(fabsf(hu) / hh, fabsf(hv) / hh) + cc;
+        m_s = fmx(m_s, sp);
+        s_vol += (double)hh;
+      }
+    }
+  }
+  double dummy = 0.0;
+  block_reduce4(m_s, s_vol, s_inf, bank, s_red);
+  (void)dummy;
+  if (tid == 0) {
+    if (STAGE == 1) {
+      a.part.edge[tile] = bank;
+    } else {
+      a.part.speed_max[tile] = m_s;
+      a.part.vol[tile] = s_vol;
+      a.part.infil[tile] = s_inf;
+      a.part.edge[tile] = a.part.edge[tile] + bank;
+    }
+  }
+}
+
+constexpr int kHBX = 64, kHBY = 16;
+
+int hllc_num_blocks(const rbs_state *s) {
+  return ((s->nx + kHBX - 1) / kHBX) * ((s->ny + kHBY - 1) / kHBY);
+}
+
+int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
+  if (!(s->armed & RBS_ARM_HLLC)) {
+    set_error("HLLC step on a state without momentum (arm RBS_ARM_HLLC)");
+    return RBS_E_STATE;
+  }
+  HlArgs a{};
+  a.z = s->z;
+  a.n = s->n;
+  a.h_comp = s->h_comp;
+  a.rain = s->rain;
+  a.infil = s->infil;
+  a.ctl = s->ctl;
+  a.part = s->part;
+  a.dx64 = s->dx;
+  a.pitch = s->pitch;
+  a.ny = s->ny;
+  a.nx = s->nx;
+  a.tiles_x = (s->nx + kHBX - 1) / kHBX;
+  a.n_tiles = hllc_num_blocks(s);
+  for (int e = 0; e < 4; ++e) a.bc[e] = s->bc[e];
+  int f = 0;
+  if (s->armed & RBS_ARM_COMP) f |= HL_COMP;
+  if (s->armed & RBS_ARM_RAIN) f |= HL_FIELD;
+  if (s->armed & RBS_ARM_INFIL) f |= HL_INFIL;
+  for (int e = 0; e < 4; ++e)
+    if (s->bc[e] != RBS_BC_CLOSED) f |= HL_LOSS;
+  a.flags = f;
+  if (a.n_tiles > s->part_cap) {
+    set_error("partial buffer too small");
+    return RBS_E_STATE;
+  }
+  float *h = s->h[s->cur];
+  // stage 1: U^n -> U1
+  a.h_in = h;
+  a.hu_in = s->hu;
+  a.hv_in = s->hv;
+  a.h_out = s->h1;
+  a.hu_out = s->hu1;
+  a.hv_out = s->hv1;
+  a.hn = h;
+  a.hun = s->hu;
+  a.hvn = s->hv;
+  hipLaunchKernelGGL((hllc_stage_kernel<kHBX, kHBY, 1>), dim3(a.n_tiles), dim3(kThreads), 0,
+                     s->stream, a);
+  RBS_HIP(hipGetLastError());
+  // stage 2: U1 (halo) + U^n (own cell) -> U^{n+1} in place
+  a.h_in = s->h1;
+  a.hu_in = s->hu1;
+  a.hv_in = s->hv1;
+  a.h_out = h;
+  a.hu_out = s->hu;
+  a.hv_out = s->hv;
+  hipLaunchKernelGGL((hllc_stage_kernel<kHBX, kHBY, 2>), dim3(a.n_tiles), dim3(kThreads), 0,
+                     s->stream, a);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
+
+}  // namespace rbs

@@ -64,6 +64,12 @@ struct DevCtl {
   int32_t run_scheme, run_limit, run_field, n_sync;
   double alpha, dt_max, rain_m_s, rain_duration, rain_field_sum;
   int64_t step_limit;  // device-driven run stops after this many steps in total
+  // fixed_stage edges (hllc.py:1308-1312): the prescribed surface of the next step,
+  // float32 as the kernels see it, and the curves a device-driven run evaluates.
+  float stage[4];
+  int32_t stage_n[4];
+  const double *stage_t[4];
+  const double *stage_lv[4];
 };
 
 // Per-block partial results of one step, reduced in fixed order by the finalize
@@ -147,7 +153,8 @@ int li_launch_step(rbs_state *s, bool device_scalars, double dt, double rain, in
 int reduce_speed_launch(rbs_state *s, int scheme, int mode);
 int reduce_partials_needed(const rbs_state *s);
 int finalize_launch(rbs_state *s, int mode, int scheme);
-int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale);
+int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale,
+                       const float stage[4] = nullptr);
 int hllc_launch_step(rbs_state *s, bool device_scalars, double dt, double rain,
                      double rain_scale, double t);
 int inflow_launch(rbs_state *s, bool device_scalars, double t, double dt);

@@ -39,6 +39,24 @@ __global__ void clock_reset_kernel(DevCtl *c, double t) {
   c->edge_out = c->edge_out_c = 0.0;
 }
 
+__global__ void stage_config_kernel(DevCtl *c, int n0, int n1, int n2, int n3, const double *t0,
+                                    const double *t1, const double *t2, const double *t3,
+                                    const double *l0, const double *l1, const double *l2,
+                                    const double *l3) {
+  c->stage_n[0] = n0;
+  c->stage_n[1] = n1;
+  c->stage_n[2] = n2;
+  c->stage_n[3] = n3;
+  c->stage_t[0] = t0;
+  c->stage_t[1] = t1;
+  c->stage_t[2] = t2;
+  c->stage_t[3] = t3;
+  c->stage_lv[0] = l0;
+  c->stage_lv[1] = l1;
+  c->stage_lv[2] = l2;
+  c->stage_lv[3] = l3;
+}
+
 __global__ void run_config_kernel(DevCtl *c, int scheme, int limit, int field, int n_sync,
                                   double alpha, double dt_max, double rain_m_s,
                                   double rain_duration, double field_sum, double t_stop,
@@ -273,6 +291,8 @@ int rbs_state_destroy(rbs_state *s) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &c : s->stage) {
+    delete[] c.t;
+    delete[] c.level;
     if (c.d_t) (void)hipFree(c.d_t);
     if (c.d_level) (void)hipFree(c.d_level);
   }
@@ -436,6 +456,44 @@ int rbs_li_step(rbs_state *s, double dt, double rain, int32_t limit, double rain
   rc = finalize_launch(s, FIN_EXPLICIT, RBS_SCHEME_LI);
   if (rc) return rc;
   s->speed_scheme = RBS_SCHEME_LI;
+  return RBS_OK;
+}
+
+// boundaries.py:49-70 stage_at (host, float64)
+static double stage_at_host(const StageCurve &c, double x) {
+  const int n = c.n;
+  if (x <= c.t[0]) return c.level[0];
+  if (x >= c.t[n - 1]) return c.level[n - 1];
+  for (int k = 0; k + 1 < n; ++k) {
+    const double t0 = c.t[k], t1 = c.t[k + 1];
+    if (t0 <= x && x <= t1) {
+      if (t1 == t0) return c.level[k + 1];
+      return c.level[k] + (c.level[k + 1] - c.level[k]) * (x - t0) / (t1 - t0);
+    }
+  }
+  return c.level[n - 1];
+}
+
+int rbs_hllc_step(rbs_state *s, double dt, double rain, double rain_scale, double t) {
+  if (!s) return set_error("null state"), RBS_E_ARG;
+  if (!(dt > 0.0) || !std::isfinite(dt)) return set_error("non-positive timestep %g", dt), RBS_E_RANGE;
+  if (!(s->armed & RBS_ARM_HLLC)) {
+    int rc = arm_fields(s, RBS_ARM_HLLC);  // hllc.step arms (hllc.py:1304)
+    if (rc) return rc;
+  }
+  float stage[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int e = 0; e < 4; ++e)
+    if (s->bc[e] == RBS_BC_FIXED_STAGE) {
+      if (s->stage[e].n < 1) return set_error("fixed_stage edge %d without a stage curve", e), RBS_E_STATE;
+      stage[e] = (float)stage_at_host(s->stage[e], t + 0.5 * dt);  // hllc.py:1308-1312
+    }
+  int rc = set_scalars_launch(s, t, dt, (float)rain, (float)rain_scale, stage);
+  if (rc) return rc;
+  rc = hllc_launch_step(s, false, dt, rain, rain_scale, t);
+  if (rc) return rc;
+  rc = finalize_launch(s, FIN_EXPLICIT, RBS_SCHEME_HLLC);
+  if (rc) return rc;
+  s->speed_scheme = RBS_SCHEME_HLLC;
   return RBS_OK;
 }
 
@@ -603,6 +661,19 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
                      p->rain_duration, p->rain_field_sum != 0.0 ? p->rain_field_sum : s->rain_field_sum,
                      t_stop, steps0 + max_steps);
   RBS_HIP(hipGetLastError());
+  {
+    int ns[4];
+    for (int e = 0; e < 4; ++e) {
+      ns[e] = (s->bc[e] == RBS_BC_FIXED_STAGE) ? s->stage[e].n : 0;
+      if (s->bc[e] == RBS_BC_FIXED_STAGE && ns[e] < 1)
+        return set_error("fixed_stage edge %d without a stage curve", e), RBS_E_STATE;
+    }
+    hipLaunchKernelGGL(stage_config_kernel, dim3(1), dim3(1), 0, s->stream, s->ctl, ns[0], ns[1],
+                       ns[2], ns[3], s->stage[0].d_t, s->stage[1].d_t, s->stage[2].d_t,
+                       s->stage[3].d_t, s->stage[0].d_level, s->stage[1].d_level,
+                       s->stage[2].d_level, s->stage[3].d_level);
+    RBS_HIP(hipGetLastError());
+  }
   // prime: CFL reduction of the current state + the first plan
   int rc = reduce_speed_launch(s, p->scheme, FIN_PRIME_DRIVEN);
   if (rc) return rc;

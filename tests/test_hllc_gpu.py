@@ -1,0 +1,161 @@
+"""HLLC HIP path (two fused SSP-RK2 stage kernels) against the oracle.
+
+Same tolerance as the local-inertial path: max|dh| and max|d(hu, hv)| <= 1e-5
+after the stated number of steps with identical dt sequences.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def rbs():
+    from rbs import _lib
+    from rbs.core import hllc, massbalance, state
+
+    _lib.load()
+    return type("R", (), dict(lib=_lib, hl=hllc, State=state.State,
+                              MassLedger=massbalance.MassLedger))
+
+
+def _bumpy(ny, nx, dx):
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    return (0.5 * np.sin(xx * 0.2) + 0.002 * xx * dx + 0.3 * np.cos(yy * 0.3)).astype(np.float32)
+
+
+def _cmp(g, o, tol=TOL):
+    d = [np.abs(g.h.numpy() - o.h).max(), np.abs(g.hu.numpy() - o.hu).max(),
+         np.abs(g.hv.numpy() - o.hv).max()]
+    assert max(d) <= tol, f"max|dU| {d}"
+    return d
+
+
+def _run_pair(rbs, orc, g, o, steps, alpha=0.45, dt_max=5.0, rain=0.0, t0=0.0):
+    t = t0
+    for _ in range(steps):
+        dt = rbs.hl.compute_dt(g, alpha, dt_max)
+        assert dt == orc.hllc_compute_dt(o, alpha, dt_max)
+        rbs.hl.step(g, dt, rain=rain, t=t)
+        orc.hllc_step(o, dt, rain=rain, t=t)
+        t += dt
+    return t
+
+
+def test_moving_water_on_a_bumpy_bed_closed_walls(rbs, orc):
+    orc.set_threads(8)
+    z = _bumpy(70, 150, 10.0)
+    yy, xx = np.mgrid[0:70, 0:150]
+    d0 = np.maximum(0.5 + 0.4 * np.sin(xx * 0.13 + yy * 0.21) - 0.2 * z, 0.0).astype(np.float32)
+    g = rbs.State.from_bed(z, 10.0, depth=d0, manning=0.03)
+    o = orc.State(z, 10.0, depth=d0, manning=0.03)
+    _run_pair(rbs, orc, g, o, 200)
+    print("\n[hllc closed] max|dU|", _cmp(g, o))
+
+
+def test_open_and_stage_edges_rain_infiltration(rbs, orc):
+    """Open west/north, time-varying stage on the east, wall south; Kahan rain field,
+    infiltration; ledger totals agree with the oracle's loss_cum."""
+    rng = np.random.default_rng(4)
+    ny, nx = 48, 96
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.01 * (nx - xx) + 0.2 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    d0 = np.maximum(0.6 - z, 0.0).astype(np.float32)
+    rain = rng.uniform(1, 3, (ny, nx)).astype(np.float32) * 1e-5
+    infil = rng.uniform(0, 5, (ny, nx)).astype(np.float32) * 1e-6
+    curve = [(0.0, 0.5), (200.0, 0.9), (400.0, 0.4)]
+    g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.035)
+    o = orc.State(z, 5.0, depth=d0, manning=0.035)
+    for s in (g, o):
+        s.set_open_boundaries({"west": "open", "north": "open", "east": "fixed_stage"},
+                              {"east": curve})
+        s.set_rain_field(rain)
+        s.set_infiltration(infil)
+        s.arm_source_compensation()
+    led_g, led_o = rbs.MassLedger.from_state(g), orc.MassLedger.from_state(o)
+    t = 0.0
+    for k in range(300):
+        dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+        assert dt == orc.hllc_compute_dt(o, 0.45, 2.0)
+        scale = 1.0 if k < 200 else 0.0
+        rbs.hl.step(g, dt, rain_scale=scale, t=t)
+        orc.hllc_step(o, dt, rain_scale=scale, t=t)
+        if scale:
+            s = float(rain.astype(np.float64).sum()) * dt * 25.0
+            led_g.add_inflow(s)
+            led_o.add_inflow(s)
+        t += dt
+    d = _cmp(g, o)
+    rg, ro = led_g.record(g, t), led_o.record(o, t)
+    assert rg.outflow_cum == pytest.approx(ro.outflow_cum, rel=1e-6, abs=1e-6)
+    assert rg.rel_error < 1e-6
+    print(f"\n[hllc edges] max|dU| {d} outflow {rg.outflow_cum:.4f} mass {rg.rel_error:.2e}")
+
+
+def test_wet_bed_stoker_on_gpu(rbs, orc):
+    """validation/test_dam_break.py:93-133 for HLLC: nRMSE < 0.03, front < 0.05."""
+    nx, dx, t_end = 800, 0.5, 8.0
+    x = (np.arange(nx) + 0.5 - nx / 2.0) * dx
+    g = rbs.State.from_bed(np.zeros((1, nx)), dx, depth=np.where(x < 0, 1.0, 0.3).reshape(1, nx),
+                           manning=0.01)
+    led = rbs.MassLedger.from_state(g)
+    t = 0.0
+    while t < t_end - 1e-9:
+        dt = min(rbs.hl.compute_dt(g, 0.45, t_end), t_end - t)
+        rbs.hl.step(g, dt)
+        t += dt
+    rec = led.record(g, t)
+    h = g.h.numpy()[0].astype(np.float64)
+    h_ref, _ = orc.stoker(x, t_end, 1.0, 0.3)
+    inter = np.abs(x) < 0.9 * 200
+    nrmse = math.sqrt(np.mean((h[inter] - h_ref[inter]) ** 2)) / 0.7
+    lvl = 0.5 * (0.3 + orc.stoker_star_depth(1.0, 0.3))
+    fr = orc.stoker_front_position(t_end, 1.0, 0.3)
+    front = abs(x[h > lvl].max() - fr) / fr
+    assert nrmse < 0.03 and front < 0.05 and rec.rel_error < 1e-6
+    assert round(nrmse, 4) == 0.0076  # M4-fidelity-step.md:197
+
+
+def test_lake_at_rest_on_gpu(rbs):
+    """validation/test_hllc_2d.py:36-55 (gate 1e-4); the achieved residual is printed."""
+    z = _bumpy(30, 40, 10.0)
+    eta0 = float(z.max()) + 2.0
+    g = rbs.State.from_bed(z, 10.0, depth=np.maximum(eta0 - z, 0.0), manning=0.03)
+    for _ in range(200):
+        rbs.hl.step(g, rbs.hl.compute_dt(g, 0.45, 5.0))
+    u, v = g.velocities_numpy()
+    worst = max(np.abs(u).max(), np.abs(v).max())
+    print(f"\n[hllc lake at rest] max|u,v| = {worst:.2e} m/s")
+    assert worst < 1e-4
+
+
+def test_device_driven_hllc_matches_host_loop(rbs):
+    from rbs.driver import DeviceRun
+    from rbs.scheduler import MultiRateScheduler
+
+    z = _bumpy(40, 64, 5.0)
+    yy, xx = np.mgrid[0:40, 0:64]
+    d0 = np.maximum(0.8 + 0.3 * np.sin(xx * 0.3) - 0.3 * z, 0).astype(np.float32)
+    curve = [(0.0, 0.9), (60.0, 1.2)]
+    sched = MultiRateScheduler(end_time=120.0, output_every=40.0, events=[60.0])
+    host = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+    dev = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+    for s in (host, dev):
+        s.set_open_boundaries({"east": "fixed_stage", "west": "open"}, {"east": curve})
+    n_host = 0
+    for tick in sched.ticks(lambda: rbs.hl.compute_dt(host, 0.45, 10.0)):
+        rbs.hl.step(host, tick.dt, t=tick.t0)
+        n_host += 1
+    run = DeviceRun(dev, "hllc_fv", sched, alpha=0.45, dt_max=10.0)
+    n_dev = sum(run.run_until(seg.t_stop) for seg in sched.segments())
+    assert n_dev == n_host
+    assert np.array_equal(host.h.numpy(), dev.h.numpy())
+    assert np.array_equal(host.hu.numpy(), dev.hu.numpy())
+    assert host.loss_volume() == pytest.approx(dev.loss_volume(), rel=1e-12)
