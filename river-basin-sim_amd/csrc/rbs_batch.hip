@@ -1,0 +1,240 @@
+// rbs_batch.hip -- many local-inertial scenarios advanced by one launch per step.
+//
+// The reference runs one scenario per process (run.py:140).  A batch holds up to
+// `capacity` states of any grid sizes; each step is one batched fused-LI launch
+// over the flat tile space of all members (a tile table maps every block back to
+// its member and tile) and one batched finalize with a block per member, so every
+// member keeps its own clock, CFL dt, sync table and float64 ledger.  Members that
+// reached their stop time idle: their tiles return at once.  Steps are replayed
+// from a captured HIP graph; adding or removing a member rebuilds the tables and
+// the graph between replays, which is what continuous batching needs.
+#include <algorithm>
+#include <vector>
+
+#include "rbs_device.hpp"
+
+using namespace rbs;
+
+struct rbs_batch {
+  int capacity = 0;
+  std::vector<rbs_state *> slot;  // slot -> member (nullptr = free)
+  hipStream_t stream = nullptr;
+  int variant = 0;
+  int chunk = 16;
+  // device tables (rebuilt when membership changes)
+  void *d_args = nullptr;      // LiArgs[capacity]
+  FinArgs *d_fin = nullptr;    // [capacity]
+  int32_t *d_status = nullptr; // [capacity]
+  int32_t *h_status = nullptr; // pinned mirror
+  int *d_tile_member = nullptr, *d_tile_first = nullptr;
+  int tile_cap = 0;
+  int total_tiles = 0;
+  std::vector<int> order;      // compact index -> slot
+  std::vector<int> active;     // per slot: 1 running, 0 done
+  hipGraphExec_t graph = nullptr;
+  bool dirty = true;
+};
+
+static void drop_graph(rbs_batch *b) {
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+}
+
+static int rebuild(rbs_batch *b) {
+  drop_graph(b);
+  b->order.clear();
+  for (int k = 0; k < b->capacity; ++k)
+    if (b->slot[k] && b->active[k]) b->order.push_back(k);
+  const int n = (int)b->order.size();
+  b->total_tiles = 0;
+  b->dirty = false;
+  if (n == 0) return RBS_OK;
+  const size_t asz = li_args_size();
+  std::vector<char> args(asz * n);
+  std::vector<FinArgs> fin(n);
+  std::vector<int> first(n), member;
+  for (int i = 0; i < n; ++i) {
+    rbs_state *s = b->slot[b->order[i]];
+    s->li_variant = b->variant;
+    li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
+    const int nt = li_num_blocks(s);
+    first[i] = b->total_tiles;
+    for (int t = 0; t < nt; ++t) member.push_back(i);
+    b->total_tiles += nt;
+    fin[i].ctl = s->ctl;
+    fin[i].part = s->part;
+    fin[i].sync = s->d_sync;
+    fin[i].area = s->dx * s->dx;
+    fin[i].n_cells = (double)s->ny * (double)s->nx;
+    fin[i].dx = s->dx;
+    fin[i].nblk = nt;
+  }
+  if (b->total_tiles > b->tile_cap) {
+    if (b->d_tile_member) RBS_HIP(hipFree(b->d_tile_member));
+    b->d_tile_member = nullptr;
+    RBS_HIP(hipMalloc(&b->d_tile_member, sizeof(int) * b->total_tiles));
+    b->tile_cap = b->total_tiles;
+  }
+  RBS_HIP(hipMemcpyAsync(b->d_args, args.data(), asz * n, hipMemcpyHostToDevice, b->stream));
+  RBS_HIP(hipMemcpyAsync(b->d_fin, fin.data(), sizeof(FinArgs) * n, hipMemcpyHostToDevice, b->stream));
+  RBS_HIP(hipMemcpyAsync(b->d_tile_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
+  RBS_HIP(hipMemcpyAsync(b->d_tile_member, member.data(), sizeof(int) * member.size(),
+                         hipMemcpyHostToDevice, b->stream));
+  RBS_HIP(hipStreamSynchronize(b->stream));
+  return RBS_OK;
+}
+
+static int capture(rbs_batch *b) {
+  const int n = (int)b->order.size();
+  RBS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = RBS_OK;
+  for (int k = 0; k < b->chunk && rc == RBS_OK; ++k) {
+    // point sources of the members that have them (inflow.py:167-200), per member
+    for (int i = 0; i < n && rc == RBS_OK; ++i) {
+      rbs_state *s = b->slot[b->order[i]];
+      if (s->inflow.count > 0) {
+        hipStream_t keep = s->stream;
+        s->stream = b->stream;
+        rc = inflow_launch(s, true, 0.0, 0.0);
+        s->stream = keep;
+      }
+    }
+    if (rc == RBS_OK)
+      rc = li_batch_launch(b->variant, (const LiArgs *)b->d_args, b->d_tile_member,
+                           b->d_tile_first, b->total_tiles, b->stream);
+    if (rc == RBS_OK) rc = batch_finalize_launch(b->d_fin, n, b->d_status, b->stream);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  return RBS_OK;
+}
+
+extern "C" {
+
+int rbs_batch_create(int32_t capacity, rbs_batch **out) {
+  if (!out || capacity < 1 || capacity > 65536) return set_error("bad batch capacity"), RBS_E_ARG;
+  rbs_batch *b = new rbs_batch();
+  b->capacity = capacity;
+  b->slot.assign(capacity, nullptr);
+  b->active.assign(capacity, 0);
+  auto fail = [&](int rc) {
+    rbs_batch_destroy(b);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(hip_fail(hipErrorUnknown, "hipStreamCreate"));
+  if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
+      hipMalloc(&b->d_fin, sizeof(FinArgs) * capacity) != hipSuccess ||
+      hipMalloc(&b->d_status, sizeof(int32_t) * capacity) != hipSuccess ||
+      hipMalloc(&b->d_tile_first, sizeof(int) * capacity) != hipSuccess)
+    return fail(hip_fail(hipErrorOutOfMemory, "hipMalloc(batch tables)"));
+  if (hipHostMalloc(&b->h_status, sizeof(int32_t) * capacity) != hipSuccess)
+    return fail(hip_fail(hipErrorOutOfMemory, "hipHostMalloc(batch status)"));
+  *out = b;
+  return RBS_OK;
+}
+
+int rbs_batch_destroy(rbs_batch *b) {
+  if (!b) return RBS_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  void *p[] = {b->d_args, b->d_fin, b->d_status, b->d_tile_member, b->d_tile_first};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  if (b->h_status) (void)hipHostFree(b->h_status);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return RBS_OK;
+}
+
+int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t_stop,
+                  int32_t *slot) {
+  if (!b || !st || !p || !slot) return set_error("null argument"), RBS_E_ARG;
+  if (p->scheme != RBS_SCHEME_LI) return set_error("a batch advances local-inertial scenarios"), RBS_E_ARG;
+  if (!p->limit) return set_error("batched steps always run the donor limiter"), RBS_E_ARG;
+  for (int k = 0; k < b->capacity; ++k)
+    if (b->slot[k] == st) return set_error("state is already in the batch"), RBS_E_STATE;
+  int k = 0;
+  while (k < b->capacity && b->slot[k]) ++k;
+  if (k == b->capacity) return set_error("batch is full (%d members)", b->capacity), RBS_E_NOMEM;
+  RBS_HIP(hipStreamSynchronize(b->stream));
+  int64_t steps0 = 0;
+  int rc = driven_setup(st, p, t_stop, 0, &steps0);
+  if (rc) return rc;
+  RBS_HIP(hipStreamSynchronize(st->stream));
+  b->slot[k] = st;
+  b->active[k] = 1;
+  b->dirty = true;
+  *slot = k;
+  return RBS_OK;
+}
+
+int rbs_batch_remove(rbs_batch *b, int32_t slot) {
+  if (!b || slot < 0 || slot >= b->capacity || !b->slot[slot])
+    return set_error("no member in slot %d", slot), RBS_E_ARG;
+  RBS_HIP(hipStreamSynchronize(b->stream));
+  rbs_state *s = b->slot[slot];
+  RBS_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->stream));
+  RBS_HIP(hipStreamSynchronize(s->stream));
+  s->cur = s->ctl_host->cur;
+  s->speed_scheme = RBS_SCHEME_LI;
+  b->slot[slot] = nullptr;
+  b->active[slot] = 0;
+  b->dirty = true;
+  return RBS_OK;
+}
+
+int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
+  if (!b || steps < 1) return set_error("bad arguments"), RBS_E_ARG;
+  if (b->dirty) {
+    int rc = rebuild(b);
+    if (rc) return rc;
+  }
+  const int n = (int)b->order.size();
+  if (n > 0) {
+    if (!b->graph) {
+      int rc = capture(b);
+      if (rc) return rc;
+    }
+    const int launches = (steps + b->chunk - 1) / b->chunk;
+    for (int L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(b->graph, b->stream));
+    RBS_HIP(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                           b->stream));
+    RBS_HIP(hipStreamSynchronize(b->stream));
+    bool changed = false;
+    for (int i = 0; i < n; ++i) {
+      const int k = b->order[i];
+      const int32_t w = b->h_status[i];
+      b->slot[k]->cur = (w >> 1) & 1;
+      if (w & 4) return set_error("member %d: non-positive timestep", k), RBS_E_RANGE;
+      if (!(w & 1)) {
+        b->active[k] = 0;
+        changed = true;
+      }
+    }
+    if (changed) b->dirty = true;  // finished members drop out of the tile space
+  }
+  int c = 0;
+  for (int k = 0; k < b->capacity; ++k) c += (b->slot[k] && b->active[k]) ? 1 : 0;
+  if (n_active) *n_active = c;
+  return RBS_OK;
+}
+
+int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active) {
+  if (!b || !active || slot < 0 || slot >= b->capacity || !b->slot[slot])
+    return set_error("no member in slot %d", slot), RBS_E_ARG;
+  *active = b->active[slot];
+  return RBS_OK;
+}
+
+}  // extern "C"

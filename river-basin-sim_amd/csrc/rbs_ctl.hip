@@ -127,10 +127,8 @@ __device__ void plan_next(DevCtl *ctl, const double *sync, double dx) {
 // Reduce the step's (or prime's) partials in fixed order and update clock + ledger.
 // mode: FIN_PRIME (no step was taken), FIN_EXPLICIT (host-driven step),
 // FIN_DRIVEN (device-driven step; plans the next one).
-__global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partials part, int nblk,
-                                                            double area, double n_cells,
-                                                            const double *sync, double dx,
-                                                            int mode) {
+__device__ void finalize_body(DevCtl *ctl, Partials part, int nblk, double area, double n_cells,
+                              const double *sync, double dx, int mode, int flip) {
   __shared__ double s_red[4 * (kThreads / 64)];
   if (mode == FIN_DRIVEN && ctl->active != 1) return;
   float m = 0.0f;
@@ -165,8 +163,33 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partial
   }
   ctl->t = ctl->t + dt;
   ctl->steps += 1;
+  if (flip) ctl->cur ^= 1;  // the LI step wrote the other buffer
   if (mode == FIN_DRIVEN) plan_next(ctl, sync, dx);
   else ctl->active = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partials part, int nblk,
+                                                            double area, double n_cells,
+                                                            const double *sync, double dx,
+                                                            int mode, int flip) {
+  finalize_body(ctl, part, nblk, area, n_cells, sync, dx, mode, flip);
+}
+
+// One block per batch member: that member's device-driven finalize, then its
+// status word (bit 0: still active, bit 1: current LI buffer) for the host.
+__global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const FinArgs *fin,
+                                                                  int32_t *status) {
+  const FinArgs f = fin[blockIdx.x];
+  finalize_body(f.ctl, f.part, f.nblk, f.area, f.n_cells, f.sync, f.dx, FIN_DRIVEN, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) status[blockIdx.x] = (f.ctl->active == 1 ? 1 : 0) | (f.ctl->cur << 1) |
+                                             (f.ctl->active == -1 ? 4 : 0);
+}
+
+int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3(n), dim3(kThreads), 0, stream, fin, status);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
 }
 
 int ensure_partials(rbs_state *s, int nblk) {
@@ -206,7 +229,7 @@ int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
   RBS_HIP(hipGetLastError());
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
                      nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx,
-                     mode);
+                     mode, 0);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }
@@ -214,7 +237,8 @@ int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
 int finalize_launch(rbs_state *s, int mode, int scheme) {
   const int nblk = scheme == RBS_SCHEME_HLLC ? hllc_num_blocks(s) : li_num_blocks(s);
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
-                     nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx, mode);
+                     nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx, mode,
+                     scheme == RBS_SCHEME_LI ? 1 : 0);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

@@ -48,6 +48,8 @@ struct DevCtl {
   double t_stop;   // device-driven run target
   int32_t active;  // 1 while the next step is due
   int32_t sync_idx;
+  int32_t cur;     // LI ping-pong buffer that holds the current state
+  int32_t pad0;
   int64_t steps;
   // per-step forcing (run.py:378-388)
   float rain;        // uniform rate handed to the step (float32)
@@ -79,6 +81,16 @@ struct Partials {
   double *vol;        // [nblk] sum f64(h_new)
   double *infil;      // [nblk] sum infiltration depth removed
   double *edge;       // [nblk] sum edge drain / banked depth (signed)
+};
+
+// What the batched finalize needs of one member.
+struct FinArgs {
+  DevCtl *ctl;
+  Partials part;
+  const double *sync;
+  double area, n_cells, dx;
+  int32_t nblk;
+  int32_t pad;
 };
 
 struct StageCurve {
@@ -153,6 +165,7 @@ int li_launch_step(rbs_state *s, bool device_scalars, double dt, double rain, in
 int reduce_speed_launch(rbs_state *s, int scheme, int mode);
 int reduce_partials_needed(const rbs_state *s);
 int finalize_launch(rbs_state *s, int mode, int scheme);
+struct LiArgs;
 int set_scalars_launch(rbs_state *s, double t, double dt, float rain, float rain_scale,
                        const float stage[4] = nullptr);
 int hllc_launch_step(rbs_state *s, bool device_scalars, double dt, double rain,
@@ -163,6 +176,14 @@ int li_num_blocks(const rbs_state *s);
 int hllc_num_blocks(const rbs_state *s);
 int li_variant_count();
 int ensure_partials(rbs_state *s, int nblk);
+
+int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
+                 int64_t *steps0);
+int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream);
+int li_batch_launch(int variant, const LiArgs *members, const int *tile_member,
+                    const int *tile_first, int total, hipStream_t stream);
+size_t li_args_size();
+void li_fill_args(const rbs_state *s, int limit, LiArgs *out);
 
 // finalize modes
 enum { FIN_PRIME = 0, FIN_EXPLICIT = 1, FIN_DRIVEN = 2, FIN_PRIME_DRIVEN = 3 };

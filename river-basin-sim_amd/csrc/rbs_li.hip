@@ -39,12 +39,9 @@ enum : int {
 };
 
 struct LiArgs {
-  const float *h_in;
-  float *h_out;
-  const float *qx_in;
-  float *qx_out;
-  const float *qy_in;
-  float *qy_out;
+  float *h[2];   // ping-pong pair; the control block's `cur` names the current one
+  float *qx[2];
+  float *qy[2];
   const float *z;
   const float *n;
   float *h_comp;
@@ -62,9 +59,16 @@ struct LiArgs {
 };
 
 template <int BX, int BY>
-__global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
+__device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   const DevCtl *ctl = a.ctl;
-  if (!ctl->active) return;  // device-driven run already at its stop time
+  if (ctl->active != 1) return;  // device-driven run already at its stop time
+  const int cur = ctl->cur;
+  const float *__restrict__ h_in = a.h[cur];
+  float *__restrict__ h_out = a.h[cur ^ 1];
+  const float *__restrict__ qx_in = a.qx[cur];
+  float *__restrict__ qx_out = a.qx[cur ^ 1];
+  const float *__restrict__ qy_in = a.qy[cur];
+  float *__restrict__ qy_out = a.qy[cur ^ 1];
 
   constexpr int EW = BX + 4, EH = BY + 4;  // cells two out: eta, z, n
   constexpr int RW = BX + 2, RH = BY + 2;  // cells one out: h, beta
@@ -79,7 +83,6 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
   __shared__ float s_beta[RH][RW];
   __shared__ double s_red[4 * (kThreads / 64)];
 
-  const int tile = xcd_tile(blockIdx.x, a.n_tiles);
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int y0 = ty * BY, x0 = tx * BX;
   const int ny = a.ny, nx = a.nx;
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
       const int lr = k / EW, lc = k - (k / EW) * EW;
       const int gi = clampi(y0 + lr - 2, 0, ny - 1), gj = clampi(x0 + lc - 2, 0, nx - 1);
       const int64_t o = (int64_t)gi * P + gj;
-      const float hv = a.h_in[o], zv = a.z[o];
+      const float hv = h_in[o], zv = a.z[o];
       s_eta[lr][lc] = hv + zv;  // compute_eta (:61-69)
       s_z[lr][lc] = zv;
       s_n[lr][lc] = a.n[o];
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
     if (k < RH * QXW) {
       const int lr = k / QXW, lf = k - (k / QXW) * QXW;
       const int gi = clampi(y0 + lr - 1, 0, ny - 1), gf = clampi(x0 + lf - 1, 0, nx);
-      s_qx[lr][lf] = a.qx_in[(int64_t)gi * P + gf];
+      s_qx[lr][lf] = qx_in[(int64_t)gi * P + gf];
     }
   }
 #pragma unroll
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
     if (k < QYH * RW) {
       const int lf = k / RW, lc = k - (k / RW) * RW;
       const int gf = clampi(y0 + lf - 1, 0, ny), gj = clampi(x0 + lc - 1, 0, nx - 1);
-      s_qy[lf][lc] = a.qy_in[(int64_t)gf * P + gj];
+      s_qy[lf][lc] = qy_in[(int64_t)gf * P + gj];
     }
   }
   __syncthreads();
@@ -275,11 +278,11 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
         h = drain(av, fmx(qyT, 0.0f), dxdt);
         s_edge += (double)av - (double)h;
       }
-      a.h_out[o] = h;
-      a.qx_out[o] = qxL;
-      if (c == nx - 1) a.qx_out[o + 1] = qxR;
-      a.qy_out[o] = qyT;
-      if (r == ny - 1) a.qy_out[o + P] = qyB;
+      h_out[o] = h;
+      qx_out[o] = qxL;
+      if (c == nx - 1) qx_out[o + 1] = qxR;
+      qy_out[o] = qyT;
+      if (r == ny - 1) qy_out[o + P] = qyB;
       m_h = fmx(m_h, h);
       s_vol += (double)h;
     }
@@ -294,22 +297,50 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
   }
 }
 
+// One state: one block per tile.
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
+  li_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
+}
+
+// A batch of states in one launch: the flat tile space of all members, mapped back
+// to (member, tile) by a table; each member keeps its own clock, dt and ledger.
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__restrict__ members,
+                                                            const int *__restrict__ tile_member,
+                                                            const int *__restrict__ tile_first,
+                                                            int total) {
+  const int t = xcd_tile(blockIdx.x, total);
+  const int m = tile_member[t];
+  const LiArgs a = members[m];
+  li_tile<BX, BY>(a, t - tile_first[m]);
+}
+
 // ---- tile variants (the autotuner picks one per grid shape) ----------------------
 struct LiVariant {
   int bx, by;
   void (*kern)(LiArgs);
+  void (*batch)(const LiArgs *, const int *, const int *, int);
 };
+#define RBS_LI_VARIANT(X, Y) {X, Y, li_step_kernel<X, Y>, li_batch_kernel<X, Y>}
 static const LiVariant kLiVariants[] = {
-    {64, 16, li_step_kernel<64, 16>},
-    {64, 8, li_step_kernel<64, 8>},
-    {64, 32, li_step_kernel<64, 32>},
-    {128, 8, li_step_kernel<128, 8>},
-    {128, 16, li_step_kernel<128, 16>},
-    {32, 32, li_step_kernel<32, 32>},
+    RBS_LI_VARIANT(64, 16), RBS_LI_VARIANT(64, 8),   RBS_LI_VARIANT(64, 32),
+    RBS_LI_VARIANT(128, 8), RBS_LI_VARIANT(128, 16), RBS_LI_VARIANT(32, 32),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 
 int li_variant_count() { return kNumLiVariants; }
+
+size_t li_args_size() { return sizeof(LiArgs); }
+
+int li_batch_launch(int variant, const LiArgs *members, const int *tile_member,
+                    const int *tile_first, int total, hipStream_t stream) {
+  if (variant < 0 || variant >= kNumLiVariants) variant = 0;
+  hipLaunchKernelGGL(kLiVariants[variant].batch, dim3(total), dim3(kThreads), 0, stream, members,
+                     tile_member, tile_first, total);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
 
 static const LiVariant &pick(const rbs_state *s) {
   int v = s->li_variant;
@@ -323,16 +354,30 @@ int li_num_blocks(const rbs_state *s) {
   return tx * ty;
 }
 
+void li_fill_args(const rbs_state *s, int limit, LiArgs *out);
+
 int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
   const LiVariant &v = pick(s);
   LiArgs a{};
-  const int cur = s->cur, nxt = 1 - s->cur;
-  a.h_in = s->h[cur];
-  a.h_out = s->h[nxt];
-  a.qx_in = s->qx[cur];
-  a.qx_out = s->qx[nxt];
-  a.qy_in = s->qy[cur];
-  a.qy_out = s->qy[nxt];
+  li_fill_args(s, limit, &a);
+  if (a.n_tiles > s->part_cap) {
+    set_error("partial buffer too small (%d < %d)", s->part_cap, a.n_tiles);
+    return RBS_E_STATE;
+  }
+  hipLaunchKernelGGL(v.kern, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
+  RBS_HIP(hipGetLastError());
+  s->cur ^= 1;
+  return RBS_OK;
+}
+
+void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
+  const LiVariant &v = pick(s);
+  LiArgs a{};
+  for (int k = 0; k < 2; ++k) {
+    a.h[k] = s->h[k];
+    a.qx[k] = s->qx[k];
+    a.qy[k] = s->qy[k];
+  }
   a.z = s->z;
   a.n = s->n;
   a.h_comp = s->h_comp;
@@ -362,14 +407,7 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
   if (s->bc[RBS_EDGE_NORTH] == RBS_BC_OPEN) f |= LI_OPEN_N;
   if (s->bc[RBS_EDGE_SOUTH] == RBS_BC_OPEN) f |= LI_OPEN_S;
   a.flags = f;
-  if (a.n_tiles > s->part_cap) {
-    set_error("partial buffer too small (%d < %d)", s->part_cap, a.n_tiles);
-    return RBS_E_STATE;
-  }
-  hipLaunchKernelGGL(v.kern, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
-  RBS_HIP(hipGetLastError());
-  s->cur = nxt;
-  return RBS_OK;
+  *out = a;
 }
 
 }  // namespace rbs

@@ -603,9 +603,8 @@ static int enqueue_driven_step(rbs_state *s, int scheme, int limit) {
   return finalize_launch(s, FIN_DRIVEN, scheme);
 }
 
-static int build_graph(rbs_state *s, int scheme, int limit, int parity, int nsteps) {
+static int build_graph(rbs_state *s, int scheme, int limit, int nsteps) {
   const int saved = s->cur;
-  s->cur = parity;
   RBS_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBS_OK;
   for (int k = 0; k < nsteps && rc == RBS_OK; ++k) rc = enqueue_driven_step(s, scheme, limit);
@@ -617,7 +616,7 @@ static int build_graph(rbs_state *s, int scheme, int limit, int parity, int nste
     return rc;
   }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-  e = hipGraphInstantiate(&s->graph[parity], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&s->graph[0], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   return RBS_OK;
@@ -625,8 +624,11 @@ static int build_graph(rbs_state *s, int scheme, int limit, int parity, int nste
 
 }  // namespace rbs
 
-extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_stop,
-                             int64_t max_steps, int64_t *steps_taken) {
+namespace rbs {
+// Everything a device-driven run needs before its first step: validation, the sync
+// table, run parameters, stage curves, and the priming reduction + first plan.
+int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
+                 int64_t *steps0_out) {
   if (!s || !p) return set_error("null argument"), RBS_E_ARG;
   if (p->scheme != RBS_SCHEME_LI && p->scheme != RBS_SCHEME_HLLC)
     return set_error("unknown scheme %d", p->scheme), RBS_E_ARG;
@@ -677,15 +679,24 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
   // prime: CFL reduction of the current state + the first plan
   int rc = reduce_speed_launch(s, p->scheme, FIN_PRIME_DRIVEN);
   if (rc) return rc;
+  *steps0_out = steps0;
+  return RBS_OK;
+}
+}  // namespace rbs
+
+extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_stop,
+                             int64_t max_steps, int64_t *steps_taken) {
+  int64_t steps0 = 0;
+  if (max_steps <= 0) max_steps = INT64_MAX / 4;
+  int rc = driven_setup(s, p, t_stop, max_steps, &steps0);
+  if (rc) return rc;
   const int key = p->scheme * 2 + (p->limit ? 1 : 0);
   if (s->graph_key != key) {
     invalidate_graph(s);
     s->graph_key = key;
   }
-  // Every graph starts from the buffer s->cur: a graph that runs all its (even
-  // number of) steps ends there again, and once the run stops every later step is
-  // a no-op, so only the last launch can end on the other buffer.
-  const int parity = p->scheme == RBS_SCHEME_LI ? s->cur : 0;
+  // The step kernels read the ping-pong parity from the control block, so one
+  // captured graph serves every starting parity.
   const int chunk = s->graph_chunk;
   int64_t taken = 0;
   for (;;) {
@@ -693,6 +704,7 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
     RBS_HIP(hipStreamSynchronize(s->stream));
     const DevCtl &c = *s->ctl_host;
     taken = c.steps - steps0;
+    s->cur = c.cur;
     if (c.active == -1) {
       set_error("non-positive timestep at t=%.9g", c.t);
       rc = RBS_E_RANGE;
@@ -705,13 +717,12 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
     want = std::min<int64_t>(want, steps0 + max_steps - c.steps);
     int64_t launches = (want + chunk - 1) / chunk;
     launches = std::max<int64_t>(1, std::min<int64_t>(launches, 64));
-    if (!s->graph[parity]) {
-      rc = build_graph(s, p->scheme, p->limit, parity, chunk);
+    if (!s->graph[0]) {
+      rc = build_graph(s, p->scheme, p->limit, chunk);
       if (rc) return rc;
     }
-    for (int64_t L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(s->graph[parity], s->stream));
+    for (int64_t L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(s->graph[0], s->stream));
   }
-  if (p->scheme == RBS_SCHEME_LI) s->cur = parity ^ (int)(taken & 1);
   s->speed_scheme = p->scheme;
   if (steps_taken) *steps_taken = taken;
   return rc;
