@@ -94,3 +94,64 @@ class MassLedger:
             "mass_max_rel_error": self.max_rel_error,
             "mass_balance_series": [r.__dict__.copy() for r in self.series],
         }
+
+
+# --- Sediment (massbalance.py:161-306) ------------------------------------------------
+SEDIMENT_GATE = 1.0e-11  # massbalance.py:178
+
+
+@dataclass
+class SedimentRecord:
+    time: float
+    bed_volume: float
+    banked_volume: float
+    gross_volume: float
+    residual: float
+    rel_error: float
+
+
+@dataclass
+class SedimentLedger:
+    """Solid-volume balance of a morphology run: the domain is closed to bedload, so
+    ``residual = -(bed + banked)`` against the causal peak of the gross displaced
+    volume (massbalance.py:193-306)."""
+
+    cell_area: float
+    solid_fraction: float
+    series: list[SedimentRecord] = field(default_factory=list)
+    _peak_gross: float = 0.0
+
+    @classmethod
+    def from_state(cls, state: State) -> SedimentLedger:
+        sed = getattr(state, "sediment", None)
+        if sed is None:
+            raise ValueError("SedimentLedger needs a state armed with arm_sediment()")
+        led = cls(cell_area=state.grid.cell_area, solid_fraction=1.0 - sed.porosity)
+        led.record(state, 0.0)
+        return led
+
+    def record(self, state: State, time: float) -> SedimentRecord:
+        import numpy as np
+
+        sed = state.sediment
+        solid = self.cell_area * self.solid_fraction
+        dz = sed.dz_cum.numpy()
+        un = sed.dz_unapplied.numpy()
+        bed = float(dz.sum()) * solid
+        banked = float(un.sum()) * solid
+        gross = float(np.abs(dz).sum()) * solid
+        residual = -(bed + banked)
+        self._peak_gross = max(self._peak_gross, gross)
+        denom = max(abs(banked), self._peak_gross, 1e-12)
+        rec = SedimentRecord(float(time), bed, banked, gross, residual, abs(residual) / denom)
+        self.series.append(rec)
+        return rec
+
+    @property
+    def max_rel_error(self) -> float:
+        return max((r.rel_error for r in self.series), default=0.0)
+
+    def as_attrs(self) -> dict:
+        return {"sediment_gate": SEDIMENT_GATE,
+                "sediment_max_rel_error": self.max_rel_error,
+                "sediment_balance_series": [r.__dict__.copy() for r in self.series]}
