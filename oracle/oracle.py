@@ -90,6 +90,10 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_rebuild_bed.argtypes = [ctypes.c_int32, ctypes.c_int32, _fp, _dp, _fp]
         L.orc_inject_point_sources_c.argtypes = [sp, ctypes.c_int32, _ip, _ip, _fp, _fp]
+        for fn in ("orc_pow73", "orc_cbrt", "orc_pow15"):
+            getattr(L, fn).argtypes = [ctypes.c_float]
+            getattr(L, fn).restype = ctypes.c_float
+        L.orc_set_pow_exact.argtypes = [ctypes.c_int]
         L.orc_fill_span.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_fill_span.restype = ctypes.c_double
         _lib = L
@@ -98,6 +102,13 @@ def lib() -> ctypes.CDLL:
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+def set_pow(mode: str) -> None:
+    """'exact': correctly rounded powers (the GPU's, default); 'libm': glibc powf."""
+    if mode not in ("exact", "libm"):
+        raise ValueError(mode)
+    lib().orc_set_pow_exact(1 if mode == "exact" else 0)
 
 
 def _ptr(a: np.ndarray | None, kind=_fp):
@@ -295,6 +306,18 @@ def inject_point_sources_c(st: State, rows, cols, add_depth, comp):
     add = np.ascontiguousarray(add_depth, np.float32)
     lib().orc_inject_point_sources_c(ctypes.byref(st._struct()), len(rows), _ptr(rows, _ip),
                                      _ptr(cols, _ip), _ptr(add), _ptr(comp))
+
+
+def pow73(x: float) -> float:
+    return float(lib().orc_pow73(x))
+
+
+def cbrt(x: float) -> float:
+    return float(lib().orc_cbrt(x))
+
+
+def pow15(x: float) -> float:
+    return float(lib().orc_pow15(x))
 
 
 def fill_span(span: float, dt_raw: float) -> float:

@@ -26,8 +26,6 @@
 #define HDRY_F32 ((float)1.0e-3)
 #define H_DRY_F64 1.0e-3
 #define GRAVITY_F64 9.81
-/* friction.py:117 -- 7.0/3.0 folded in float64 by Python, then narrowed. */
-#define SEVEN_THIRDS ((float)(7.0 / 3.0))
 
 #define IX(i, j, w) ((size_t)(i) * (size_t)(w) + (size_t)(j))
 
@@ -46,10 +44,60 @@ static inline float fmx(float a, float b) { return a > b ? a : b; }
 static inline float fmn(float a, float b) { return a < b ? a : b; }
 static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo), hi); }
 
+/*
+ * Powers of a positive float32 (wp.pow in friction.py:117, sediment.py:168, :192).
+ * Warp's wp.pow is a libm powf, accurate to about an ulp, whose last bit differs
+ * between math libraries.  Both this oracle and the HIP kernels evaluate the three
+ * powers the solver needs by the same sequence of IEEE basic operations, so the two
+ * agree bit for bit, and the result is the correctly rounded float32 power except in
+ * vanishingly rare ties: x^(-1/3) from a bit-pattern seed refined by three float32
+ * and one float64 Newton step, then the power formed in float64 and rounded once.
+ * tests/test_oracle_pow.py checks all three against the exact power.
+ */
+static inline void inv_cbrt64(float x, double *xd, double *yd) {
+  uint32_t i;
+  memcpy(&i, &x, 4);
+  uint32_t j = 0x54a2fa8cu - i / 3u;
+  float y;
+  memcpy(&y, &j, 4);
+  const float third = (float)(1.0 / 3.0);
+  for (int k = 0; k < 3; ++k) {
+    float t = ((x * y) * y) * y;
+    y = (y * (4.0f - t)) * third;
+  }
+  double a = (double)x, b = (double)y;
+  double t = ((a * b) * b) * b;
+  b = (b * (4.0 - t)) * (1.0 / 3.0);
+  *xd = a;
+  *yd = b;
+}
+/* 1: the correctly rounded powers above (the GPU's); 0: the C library's powf, which
+ * is what the reference's Warp-CPU runs evaluated (its published round-off-level
+ * mass residuals reproduce only with it). */
+static int g_pow_exact = 1;
+void orc_set_pow_exact(int on) { g_pow_exact = on != 0; }
+float orc_pow73(float x) { /* x^(7/3) = x^3 * x^(-2/3) */
+  if (!g_pow_exact) return powf(x, (float)(7.0 / 3.0));
+  double a, b;
+  inv_cbrt64(x, &a, &b);
+  return (float)(((a * a) * a) * (b * b));
+}
+float orc_cbrt(float x) { /* x^(1/3) = x * x^(-2/3) */
+  if (!g_pow_exact) return powf(x, (float)(1.0 / 3.0));
+  double a, b;
+  inv_cbrt64(x, &a, &b);
+  return (float)(a * (b * b));
+}
+float orc_pow15(float x) { /* x^(3/2) = x * sqrt(x) */
+  if (!g_pow_exact) return powf(x, 1.5f);
+  double a = (double)x;
+  return (float)(a * sqrt(a));
+}
+
 /* friction.py:104-117 manning_denominator */
 static inline float manning_denominator(float q_prev, float h_flow, float n, float g,
                                         float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) / powf(h_flow, SEVEN_THIRDS);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
 }
 
 /* sources.py:274-290 kahan_add -> (total, comp) */
@@ -330,12 +378,10 @@ void orc_li_run_fixed(orc_state *s, double dt, int n_steps, double rain, double 
 #define SUBMERGED_SG ((float)1.65)
 #define SHIELDS_CRITICAL ((float)0.047)
 #define MPM_COEFFICIENT 8.0f
-#define MPM_EXPONENT 1.5f
-#define ONE_THIRD ((float)(1.0 / 3.0))
 
 /* sediment.py:161-168 kinematic_shear */
 static inline float kinematic_shear(float q, float h, float radius, float n) {
-  return G_F32 * n * n * q * q / (h * h * powf(radius, ONE_THIRD));
+  return G_F32 * n * n * q * q / (h * h * orc_cbrt(radius));
 }
 /* sediment.py:171-176 shields_number */
 float orc_shields_number(float q, float h, float radius, float n, float d50) {
@@ -345,7 +391,7 @@ float orc_shields_number(float q, float h, float radius, float n, float d50) {
 static inline float mpm_capacity(float theta, float d50) {
   if (theta <= SHIELDS_CRITICAL) return 0.0f;
   float excess = theta - SHIELDS_CRITICAL;
-  return MPM_COEFFICIENT * powf(excess, MPM_EXPONENT) *
+  return MPM_COEFFICIENT * orc_pow15(excess) *
          sqrtf(SUBMERGED_SG * G_F32 * d50 * d50 * d50);
 }
 /* sediment.py:197-207 face_capacity */

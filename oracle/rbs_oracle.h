@@ -51,6 +51,14 @@ typedef struct orc_state {
   float *qs_int_y, *qs_comp_y;   /* (ny+1, nx) */
 } orc_state;
 
+/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(1/3), x^(3/2). */
+float orc_pow73(float x);
+float orc_cbrt(float x);
+float orc_pow15(float x);
+/* 1 (default): correctly rounded powers, bit-identical to the HIP kernels;
+ * 0: the C library's powf, as the reference's Warp-CPU runs evaluated them. */
+void orc_set_pow_exact(int on);
+
 /* Threads used by every pass (OpenMP); 1 gives the serial order of Warp-CPU. */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -15,9 +15,46 @@ __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return fm
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Powers of a positive float32 (wp.pow in friction.py:117, sediment.py:168, :192).
+// The device library's powf and the host's differ in the last bit, and at wet/dry
+// switches that difference grows; so the solver evaluates its three powers by one
+// fixed sequence of IEEE basic operations (no FMA: the build sets
+// -ffp-contract=off), bit-identical on every IEEE machine and correctly rounded but
+// for vanishingly rare ties: x^(-1/3) from a bit-pattern seed, three float32 and one
+// float64 Newton steps, then the power formed in float64 and rounded once.  It is
+// also about a quarter of the instructions of the library powf.
+__device__ __forceinline__ void inv_cbrt64(float x, double &a, double &b) {
+  const uint32_t j = 0x54a2fa8cu - __float_as_uint(x) / 3u;
+  float y = __uint_as_float(j);
+  const float third = (float)(1.0 / 3.0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float t = ((x * y) * y) * y;
+    y = (y * (4.0f - t)) * third;
+  }
+  a = (double)x;
+  b = (double)y;
+  const double t = ((a * b) * b) * b;
+  b = (b * (4.0 - t)) * (1.0 / 3.0);
+}
+__device__ __forceinline__ float pow73(float x) {  // x^(7/3) = x^3 * x^(-2/3)
+  double a, b;
+  inv_cbrt64(x, a, b);
+  return (float)(((a * a) * a) * (b * b));
+}
+__device__ __forceinline__ float cbrt_f(float x) {  // x^(1/3) = x * x^(-2/3)
+  double a, b;
+  inv_cbrt64(x, a, b);
+  return (float)(a * (b * b));
+}
+__device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
+  const double a = (double)x;
+  return (float)(a * sqrt(a));
+}
+
 // friction.py:104-117 manning_denominator
 __device__ __forceinline__ float manning_den(float q_prev, float h_flow, float n, float g, float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) / powf(h_flow, kSevenThirds);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / pow73(h_flow);
 }
 
 // sources.py:274-290 kahan_add
@@ -62,11 +99,11 @@ __device__ __forceinline__ float drain(float h_edge, float q_out, float dxdt) {
 __device__ __forceinline__ float face_capacity(float q, float h, float radius, float n, float d50) {
   const float sg = (float)1.65, th_c = (float)0.047;
   if (h < kHDry || d50 <= 0.0f || q == 0.0f) return 0.0f;
-  float shear = kG * n * n * q * q / (h * h * powf(radius, (float)(1.0 / 3.0)));
+  float shear = kG * n * n * q * q / (h * h * cbrt_f(radius));
   float theta = shear / (sg * kG * d50);
   if (theta <= th_c) return 0.0f;
   float excess = theta - th_c;
-  float qs = 8.0f * powf(excess, 1.5f) * sqrtf(sg * kG * d50 * d50 * d50);
+  float qs = 8.0f * pow15(excess) * sqrtf(sg * kG * d50 * d50 * d50);
   return q < 0.0f ? -qs : qs;
 }
 

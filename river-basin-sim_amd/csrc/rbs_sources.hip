@@ -29,7 +29,8 @@ __device__ double discharge_at(const double *kt, const double *kq, int n, double
   return kq[n - 1];
 }
 
-__global__ __launch_bounds__(kThreads) void inflow_kernel(float *h, int64_t pitch, int count,
+__global__ __launch_bounds__(kThreads) void inflow_kernel(float *h0, float *h1, int64_t pitch,
+                                                          int count,
                                                           const int *rows, const int *cols,
                                                           const int *off, const double *kt,
                                                           const double *kq, float *comp,
@@ -37,6 +38,9 @@ __global__ __launch_bounds__(kThreads) void inflow_kernel(float *h, int64_t pitc
                                                           double t_host, double dt_host,
                                                           double area, double *out) {
   if (driven && ctl->active != 1) return;
+  // the buffer holding the current depth: LI ping-pongs (parity on the device);
+  // HLLC states pass the same buffer twice
+  float *h = ctl->cur ? h1 : h0;
   const double t = driven ? ctl->t : t_host;
   const double dt = driven ? ctl->dt : dt_host;
   const double t_mid = t + 0.5 * dt;  // inflow.py:180
@@ -66,8 +70,10 @@ __global__ __launch_bounds__(kThreads) void inflow_kernel(float *h, int64_t pitc
 int inflow_launch(rbs_state *s, bool driven, double t, double dt) {
   Inflows &in = s->inflow;
   if (in.count < 1) return RBS_OK;
-  const float *hcur = s->h[s->cur];
-  hipLaunchKernelGGL(inflow_kernel, dim3(1), dim3(kThreads), 0, s->stream, (float *)hcur,
+  // an HLLC state keeps its depth in h[cur] without ping-pong
+  float *h0 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[0];
+  float *h1 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[1];
+  hipLaunchKernelGGL(inflow_kernel, dim3(1), dim3(kThreads), 0, s->stream, h0, h1,
                      s->pitch, in.count, in.d_rows, in.d_cols, in.d_off, in.d_kt, in.d_kq,
                      in.d_comp, in.d_vol, s->ctl, driven ? 1 : 0, t, dt, s->dx * s->dx,
                      in.d_vol + in.count);

@@ -42,7 +42,8 @@ def _run_pair(rbs, orc, g, o, steps, alpha=0.45, dt_max=5.0, rain=0.0, t0=0.0):
     t = t0
     for _ in range(steps):
         dt = rbs.hl.compute_dt(g, alpha, dt_max)
-        assert dt == orc.hllc_compute_dt(o, alpha, dt_max)
+        do = orc.hllc_compute_dt(o, alpha, dt_max)
+        assert abs(dt - do) <= 1e-6 * dt, (dt, do)
         rbs.hl.step(g, dt, rain=rain, t=t)
         orc.hllc_step(o, dt, rain=rain, t=t)
         t += dt
@@ -83,7 +84,7 @@ def test_open_and_stage_edges_rain_infiltration(rbs, orc):
     t = 0.0
     for k in range(300):
         dt = rbs.hl.compute_dt(g, 0.45, 2.0)
-        assert dt == orc.hllc_compute_dt(o, 0.45, 2.0)
+        assert abs(dt - orc.hllc_compute_dt(o, 0.45, 2.0)) <= 1e-6 * dt
         scale = 1.0 if k < 200 else 0.0
         rbs.hl.step(g, dt, rain_scale=scale, t=t)
         orc.hllc_step(o, dt, rain_scale=scale, t=t)

@@ -36,6 +36,15 @@ def _pair(rbs, orc, bed, dx, depth, manning, *, threads=8):
     return g, o
 
 
+def _dt(rbs, orc, g, o, alpha, dt_max):
+    """The step both sides take: the device's CFL dt (states may differ by ulps, so
+    the oracle's own dt is only checked to agree closely)."""
+    dt = rbs.li.compute_dt(g, alpha, dt_max)
+    do = orc.li_compute_dt(o, alpha, dt_max)
+    assert abs(dt - do) <= 1e-6 * dt, (dt, do)
+    return dt
+
+
 def _cmp(g, o, tol=TOL):
     dh = np.abs(g.h.numpy() - o.h).max()
     dqx = np.abs(g.qx.numpy() - o.qx).max()
@@ -61,14 +70,21 @@ def test_cfg1_dambreak_1000_steps_parity_mass_and_stoker(rbs, orc):
     x = (np.arange(nx) + 0.5 - 256.0) * 1.0
     t = 250 * dt
     h_ref, _ = orc.stoker(x, t, 1.0, 0.1)
-    h = g.h.numpy()[256].astype(np.float64)
     inter = np.abs(x) < 0.9 * 256
-    nrmse = math.sqrt(np.mean((h[inter] - h_ref[inter]) ** 2)) / 0.9
     lvl = 0.5 * (0.1 + orc.stoker_star_depth(1.0, 0.1))
-    front = x[h > lvl].max()
-    front_err = abs(front - orc.stoker_front_position(t, 1.0, 0.1)) / orc.stoker_front_position(
-        t, 1.0, 0.1)
-    assert nrmse < 0.10 and front_err < 0.15  # validation/test_dam_break.py:49
+    fref = orc.stoker_front_position(t, 1.0, 0.1)
+
+    def score(row):
+        h = row.astype(np.float64)
+        nrmse = math.sqrt(np.mean((h[inter] - h_ref[inter]) ** 2)) / 0.9
+        return nrmse, abs(x[h > lvl].max() - fref) / fref
+
+    nrmse, front_err = score(g.h.numpy()[256])
+    nrmse_o, front_o = score(o.h[256])
+    # h_R/h_L = 0.1 is a far stronger bore than the reference gate's 0.3 (which the
+    # next test runs): the shape score is the oracle's own, to 1e-4
+    assert abs(nrmse - nrmse_o) < 1e-4 and abs(front_err - front_o) < 1e-4
+    assert nrmse < 0.25 and front_err < 0.15
     for k in range(750):
         rbs.li.step(g, dt)
     orc.li_run_fixed(o, dt, 750)
@@ -77,6 +93,31 @@ def test_cfg1_dambreak_1000_steps_parity_mass_and_stoker(rbs, orc):
     assert rec.rel_error < 1e-7, f"mass closure {rec.rel_error:.2e}"
     print(f"\n[cfg1 parity] max|dh|={dh:.2e} max|dq|={dq:.2e} mass={rec.rel_error:.2e} "
           f"stoker nRMSE={nrmse:.4f} front={front_err:.4f}")
+
+
+def test_li_wet_bed_stoker_gate_on_gpu(rbs, orc):
+    """validation/test_dam_break.py:93-133 for LI: nRMSE < 0.10, front < 0.15, mass < 1e-6;
+    the published run gives 0.074 / 9.5 % (M1-water-moves.md:231-233)."""
+    nx, dx, t_end = 800, 0.5, 8.0
+    x = (np.arange(nx) + 0.5 - nx / 2.0) * dx
+    g = rbs.State.from_bed(np.zeros((1, nx)), dx, depth=np.where(x < 0, 1.0, 0.3).reshape(1, nx),
+                           manning=0.01)
+    led = rbs.MassLedger.from_state(g)
+    t = 0.0
+    while t < t_end - 1e-9:
+        dt = min(rbs.li.compute_dt(g, 0.7, t_end), t_end - t)
+        rbs.li.step(g, dt)
+        t += dt
+    rec = led.record(g, t)
+    h = g.h.numpy()[0].astype(np.float64)
+    h_ref, _ = orc.stoker(x, t_end, 1.0, 0.3)
+    inter = np.abs(x) < 0.9 * 200
+    nrmse = math.sqrt(np.mean((h[inter] - h_ref[inter]) ** 2)) / 0.7
+    lvl = 0.5 * (0.3 + orc.stoker_star_depth(1.0, 0.3))
+    fr = orc.stoker_front_position(t_end, 1.0, 0.3)
+    front = abs(x[h > lvl].max() - fr) / fr
+    assert nrmse < 0.10 and front < 0.15 and rec.rel_error < 1e-6
+    assert round(nrmse, 3) == 0.074 and round(front, 3) == 0.095
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 37), (41, 1), (2, 2), (67, 133), (130, 65)])
@@ -89,8 +130,7 @@ def test_odd_shapes_match(rbs, orc, shape):
     n = rng.uniform(0.02, 0.08, shape)
     g, o = _pair(rbs, orc, z, 2.0, h, n)
     for _ in range(40):
-        dt = rbs.li.compute_dt(g, 0.7, 1.0)
-        assert dt == orc.li_compute_dt(o, 0.7, 1.0)
+        dt = _dt(rbs, orc, g, o, 0.7, 1.0)
         rbs.li.step(g, dt)
         orc.li_step(o, dt)
     _cmp(g, o, tol=1e-6)
@@ -117,8 +157,7 @@ def test_sources_sinks_open_edges_parity(rbs, orc):
     t = 0.0
     for k in range(600):
         scale = 1.0 if k < 400 else 0.0
-        dt = rbs.li.compute_dt(g, 0.7, 5.0)
-        assert dt == orc.li_compute_dt(o, 0.7, 5.0)
+        dt = _dt(rbs, orc, g, o, 0.7, 5.0)
         rbs.li.step(g, dt, rain_scale=scale)
         orc.li_step(o, dt, rain_scale=scale)
         if scale:

@@ -37,18 +37,31 @@ def _dambreak(orc, scheme, h_l, h_r, nx, dx, t_end, n, alpha):
     return x, st.h[0].astype(np.float64), led
 
 
-@pytest.mark.parametrize(
-    "scheme,alpha,nrmse_pub,front_pub,mass_pub",
-    [
-        # docs/plans/M1-water-moves.md:231-233 (LI, Warp CPU): 0.074 / 9.5 % / 2.5e-9
-        ("li", 0.7, 0.074, 0.095, 2.5e-9),
-        # docs/plans/M4-fidelity-step.md:195-199 (HLLC): 0.0076 / 0.0101 / 8.0e-10
-        ("hllc", 0.45, 0.0076, 0.0101, 8.0e-10),
-    ],
-)
-def test_wet_bed_stoker_reproduces_the_published_figures(orc, scheme, alpha, nrmse_pub,
-                                                         front_pub, mass_pub):
-    """validation/test_dam_break.py:93-133 setup; the reference's own published numbers."""
+def _published():
+    import json
+    from pathlib import Path
+
+    return json.loads((Path(__file__).parent / "golden" / "published_accuracy.json").read_text())
+
+
+@pytest.mark.parametrize("scheme,alpha,key", [("li", 0.7, "li_wet_bed_stoker"),
+                                              ("hllc", 0.45, "hllc_wet_bed_stoker")])
+def test_wet_bed_stoker_reproduces_the_published_figures(orc, scheme, alpha, key):
+    """validation/test_dam_break.py:93-133 setup; the reference's own published numbers
+    (tests/golden/published_accuracy.json)."""
+    pub = _published()[key]
+    nrmse_pub, front_pub, mass_pub = pub["nrmse"], pub["front_err"], pub["mass_rel"]
+    orc.set_pow("libm")  # the reference's Warp-CPU runs used the C library's powf
+    try:
+        _check_published(orc, scheme, alpha, nrmse_pub, front_pub, mass_pub, exact_mass=True)
+    finally:
+        orc.set_pow("exact")
+    # with the correctly rounded powers the shape scores are the same to the published
+    # digits; the mass residual is float32 round-off either way, far under the gate
+    _check_published(orc, scheme, alpha, nrmse_pub, front_pub, mass_pub, exact_mass=False)
+
+
+def _check_published(orc, scheme, alpha, nrmse_pub, front_pub, mass_pub, exact_mass):
     x, h, led = _dambreak(orc, scheme, 1.0, 0.3, 800, 0.5, 8.0, 0.01, alpha)
     h_ref, _ = orc.stoker(x, 8.0, 1.0, 0.3)
     inter = np.abs(x) < 0.9 * (800 / 2.0 * 0.5)
@@ -59,8 +72,9 @@ def test_wet_bed_stoker_reproduces_the_published_figures(orc, scheme, alpha, nrm
     # published to 2-3 significant figures
     assert round(nrmse, 4 if scheme == "hllc" else 3) == pytest.approx(nrmse_pub, abs=1e-12)
     assert front_err == pytest.approx(front_pub, abs=6e-4)
-    assert led.max_rel_error == pytest.approx(mass_pub, rel=0.05)
-    assert led.max_rel_error < orc.MASS_GATE
+    if exact_mass:
+        assert led.max_rel_error == pytest.approx(mass_pub, rel=0.05)
+    assert led.max_rel_error < 1e-8 < orc.MASS_GATE
 
 
 def test_stoker_star_depth_closed_form(orc):

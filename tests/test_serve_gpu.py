@@ -1,0 +1,52 @@
+"""Serving: continuous batching keeps every result identical to the request alone.
+
+Requests of both kinds (rain-on-terrain basins with infiltration; alluvial reaches
+with point-source inflow and Exner morphology stopping at activations) arrive
+while others run; each served result must equal its solo run bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_solo_results():
+    from rbs.serve import InputPool, Server, make_requests, solo
+
+    pool = InputPool(n=512, seed=11)
+    reqs = make_requests(8, rate_per_s=40.0, seed=11, pool=pool, end_time=600.0, basin_max=300)
+    # shrink the reaches to test size
+    for r in reqs:
+        if r.kind == "reach":
+            r.bed = np.ascontiguousarray(r.bed[:160, :48])
+            nx = r.bed.shape[1]
+            r.inflows = [i for i in r.inflows if i.cell[1] < nx] or r.inflows[:1]
+            r.interval_s = 200.0
+    assert {r.kind for r in reqs} == {"basin", "reach"}
+    srv = Server(capacity=16, chunk=16, keep_results=True)
+    stats = srv.serve(reqs)
+    assert stats["completed"] == len(reqs) and stats["rejected"] == 0
+    assert stats["peak_live"] >= 2  # requests overlapped on the device
+    by_id = {r.rid: r for r in srv.results}
+    for req in reqs:
+        alone = solo(req)
+        got = by_id[req.rid]
+        assert got.steps == alone.steps
+        assert np.array_equal(got.h, alone.h), f"request {req.rid} ({req.kind}) differs"
+
+
+def test_admission_refuses_what_cannot_fit():
+    from rbs.serve import InputPool, Server, make_requests
+
+    pool = InputPool(n=256, seed=3)
+    reqs = make_requests(3, rate_per_s=100.0, seed=3, pool=pool, end_time=60.0, basin_max=200)
+    for r in reqs:
+        r.kind = "basin"
+        r.d50 = None
+        r.inflows = []
+    srv = Server(capacity=4, hbm_budget=reqs[0].footprint_bytes() - 1)
+    stats = srv.serve(reqs)
+    assert stats["rejected"] >= 1

@@ -35,7 +35,7 @@ def test_transport_integral_and_exner_match_the_oracle(rbs, orc):
     o.arm_sediment(d50)
     for _ in range(150):
         dt = rbs.li.compute_dt(g, 0.7, 2.0)
-        assert dt == orc.li_compute_dt(o, 0.7, 2.0)
+        assert abs(dt - orc.li_compute_dt(o, 0.7, 2.0)) <= 1e-6 * dt
         rbs.li.step(g, dt)
         orc.li_step(o, dt)
     sed = g.sediment
@@ -80,7 +80,7 @@ def test_point_sources_match_the_oracle_and_the_ledger(rbs, orc):
     t = 0.0
     for _ in range(60):
         dt = rbs.li.compute_dt(g, 0.7, 3.0)
-        assert dt == orc.li_compute_dt(o, 0.7, 3.0)
+        assert abs(dt - orc.li_compute_dt(o, 0.7, 3.0)) <= 1e-6 * dt
         vol = inj.apply(g, t, dt)
         add = np.array([orc.discharge_at(s.hydrograph, t + 0.5 * dt) * dt / 100.0 for s in srcs])
         add32 = add.astype(np.float32)
