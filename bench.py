@@ -109,7 +109,10 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="li", choices=["li", "ensemble", "serve", "hllc"])
+    ap.add_argument("--workload", default="li",
+                    choices=["li", "ensemble", "serve", "hllc", "copy"])
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time every LI kernel variant on the spun-up state")
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--rate", type=float, default=4.0, help="served arrivals per second")
@@ -132,6 +135,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "hllc":
         return bench_hllc(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     st, d = build_state(args.n, args.seed)
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
@@ -174,6 +179,9 @@ def main() -> int:
     achieved = BYTES_PER_CELL * cells / (kern_ms * 1e-3) / 1e9
     log(f"[bench] fused LI kernel mean {kern_ms * 1e3:.1f} us -> {achieved:.0f} GB/s "
         f"({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    sweep = None
+    if args.sweep:
+        sweep = variant_sweep(st, dt_k, args.kernel_reps, cells, args.variant)
     traffic = None
     tf = ROOT / "profiles" / "li_rain_4096_traffic.json"
     if tf.exists() and args.n == 4096:
@@ -226,7 +234,57 @@ def main() -> int:
         "cpu_baseline": cpu,
         "mass_closure_of_throughput": mass,
     }
+    if sweep:
+        out["variant_sweep"] = sweep
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def variant_sweep(st, dt: float, reps: int, cells: int, keep: int) -> list:
+    """Mean fused-kernel duration of every LI variant on the same (advancing) state."""
+    from rbs import _lib
+
+    count = ctypes.c_int32()
+    _lib.call("rbs_li_variant_count", ctypes.byref(count))
+    rows = []
+    for v in range(count.value):
+        bx, by, fam, dflt = (ctypes.c_int32() for _ in range(4))
+        _lib.call("rbs_li_variant_info", v, ctypes.byref(bx), ctypes.byref(by),
+                  ctypes.byref(fam), ctypes.byref(dflt))
+        _lib.call("rbs_set_li_variant", st._handle, v)
+        avg = ctypes.c_double()
+        _lib.call("rbs_time_li_kernel", st._handle, reps, dt, 0.0, 1.0, ctypes.byref(avg))
+        us = avg.value * 1e3
+        gbs = BYTES_PER_CELL * cells / (us * 1e-6) / 1e9
+        fam_name = "strip" if fam.value else "lds_tile"
+        rows.append({"variant": v, "family": fam_name, "tile": f"{bx.value}x{by.value}",
+                     "default": bool(dflt.value), "kernel_us": round(us, 2),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        log(f"[bench] variant {v} {fam_name} {bx.value}x{by.value}: {us:.1f} us, "
+            f"{100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s")
+    if keep >= 0:
+        _lib.call("rbs_set_li_variant", st._handle, keep)
+    else:
+        for r in rows:
+            if r["default"]:
+                _lib.call("rbs_set_li_variant", st._handle, r["variant"])
+    return rows
+
+
+def bench_copy(args) -> int:
+    """A plain device copy of known size, for calibrating the HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE) that the traffic figure is scaled by."""
+    import torch
+
+    n = 256 * 1024 * 1024  # 1 GiB of float32
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    launches = 5
+    for _ in range(launches):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy_calibration", "bytes_read_per_launch": 4 * n,
+                      "bytes_written_per_launch": 4 * n, "launches": launches}), flush=True)
     return 0
 
 

@@ -213,10 +213,19 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active);
  * receives the mean kernel duration.  Advances the state. */
 int rbs_time_li_kernel(rbs_state *st, int32_t reps, double dt, double rain, double rain_scale,
                        double *avg_ms);
-/* Select the LI tile variant (0 = autotuned default).  Returns RBS_E_ARG for an
- * unknown variant.  Results are bitwise identical across variants. */
+/* Select the LI kernel variant, 0 <= variant < rbs_li_variant_count() (a new state
+ * starts on the library's default).  Returns RBS_E_ARG for an unknown variant.
+ * Depths and faces are bitwise identical across variants (the ledger's volume sum
+ * may differ in its last bits: its partials follow the tile shape). */
 int rbs_set_li_variant(rbs_state *st, int32_t variant);
 int rbs_li_variant_count(int32_t *count);
+/* Shape of a variant: the block's tile of cells (bx x by) and its family
+ * (0 = LDS tile, 1 = register strips); *is_default = 1 for the library default. */
+int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *family,
+                        int32_t *is_default);
+/* The LI kernel variant a batch launches for all its members (the default unless
+ * set); takes effect at the next advance. */
+int rbs_batch_set_variant(rbs_batch *b, int32_t variant);
 
 #ifdef __cplusplus
 }

@@ -19,8 +19,9 @@ struct rbs_batch {
   int capacity = 0;
   std::vector<rbs_state *> slot;  // slot -> member (nullptr = free)
   hipStream_t stream = nullptr;
-  int variant = 0;
+  int variant = li_default_variant();
   int chunk = 16;
+  bool any_sed = false;  // a member carries the bedload integral
   // device tables (rebuilt when membership changes)
   void *d_args = nullptr;      // LiArgs[capacity]
   FinArgs *d_fin = nullptr;    // [capacity]
@@ -49,6 +50,7 @@ static int rebuild(rbs_batch *b) {
     if (b->slot[k] && b->active[k]) b->order.push_back(k);
   const int n = (int)b->order.size();
   b->total_tiles = 0;
+  b->any_sed = false;
   b->dirty = false;
   if (n == 0) return RBS_OK;
   const size_t asz = li_args_size();
@@ -58,6 +60,7 @@ static int rebuild(rbs_batch *b) {
   for (int i = 0; i < n; ++i) {
     rbs_state *s = b->slot[b->order[i]];
     s->li_variant = b->variant;
+    if (s->armed & RBS_ARM_SED) b->any_sed = true;
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
     first[i] = b->total_tiles;
@@ -102,7 +105,7 @@ static int capture(rbs_batch *b) {
       }
     }
     if (rc == RBS_OK)
-      rc = li_batch_launch(b->variant, (const LiArgs *)b->d_args, b->d_tile_member,
+      rc = li_batch_launch(b->variant, b->any_sed, (const LiArgs *)b->d_args, b->d_tile_member,
                            b->d_tile_first, b->total_tiles, b->stream);
     if (rc == RBS_OK) rc = batch_finalize_launch(b->d_fin, n, b->d_status, b->stream);
   }
@@ -234,6 +237,17 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active) {
   if (!b || !active || slot < 0 || slot >= b->capacity || !b->slot[slot])
     return set_error("no member in slot %d", slot), RBS_E_ARG;
   *active = b->active[slot];
+  return RBS_OK;
+}
+
+int rbs_batch_set_variant(rbs_batch *b, int32_t variant) {
+  if (!b) return set_error("null batch"), RBS_E_ARG;
+  if (variant < 0 || variant >= li_variant_count())
+    return set_error("LI variant %d is not in [0, %d)", variant, li_variant_count()), RBS_E_ARG;
+  if (variant != b->variant) {
+    b->variant = variant;
+    b->dirty = true;  // tables and graph follow the new tile shape
+  }
   return RBS_OK;
 }
 

@@ -127,19 +127,41 @@ __device__ void plan_next(DevCtl *ctl, const double *sync, double dx) {
 // Reduce the step's (or prime's) partials in fixed order and update clock + ledger.
 // mode: FIN_PRIME (no step was taken), FIN_EXPLICIT (host-driven step),
 // FIN_DRIVEN (device-driven step; plans the next one).
+// Thread t folds partials t, t + NT, t + 2 NT, ... in that order (four loads in
+// flight per round), then the block reduces the NT results in a fixed tree.
+template <int NT>
 __device__ void finalize_body(DevCtl *ctl, Partials part, int nblk, double area, double n_cells,
                               const double *sync, double dx, int mode, int flip) {
-  __shared__ double s_red[4 * (kThreads / 64)];
+  __shared__ double s_red[4 * (NT / 64)];
   if (mode == FIN_DRIVEN && ctl->active != 1) return;
   float m = 0.0f;
   double vol = 0.0, inf = 0.0, edge = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kThreads) {
+  int b = threadIdx.x;
+  for (; b + 3 * NT < nblk; b += 4 * NT) {
+    float mk[4];
+    double vk[4], ik[4], ek[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      mk[j] = part.speed_max[b + j * NT];
+      vk[j] = part.vol[b + j * NT];
+      ik[j] = part.infil[b + j * NT];
+      ek[j] = part.edge[b + j * NT];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      m = fmx(m, mk[j]);
+      vol += vk[j];
+      inf += ik[j];
+      edge += ek[j];
+    }
+  }
+  for (; b < nblk; b += NT) {
     m = fmx(m, part.speed_max[b]);
     vol += part.vol[b];
     inf += part.infil[b];
     edge += part.edge[b];
   }
-  block_reduce4(m, vol, inf, edge, s_red);
+  block_reduce4<NT>(m, vol, inf, edge, s_red);
   if (threadIdx.x != 0) return;
   ctl->speed_max = m;
   ctl->stale = 0;
@@ -168,11 +190,13 @@ __device__ void finalize_body(DevCtl *ctl, Partials part, int nblk, double area,
   else ctl->active = 0;
 }
 
-__global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partials part, int nblk,
-                                                            double area, double n_cells,
-                                                            const double *sync, double dx,
-                                                            int mode, int flip) {
-  finalize_body(ctl, part, nblk, area, n_cells, sync, dx, mode, flip);
+constexpr int kFinThreads = 1024;
+
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(DevCtl *ctl, Partials part, int nblk,
+                                                               double area, double n_cells,
+                                                               const double *sync, double dx,
+                                                               int mode, int flip) {
+  finalize_body<kFinThreads>(ctl, part, nblk, area, n_cells, sync, dx, mode, flip);
 }
 
 // One block per batch member: that member's device-driven finalize, then its
@@ -180,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(DevCtl *ctl, Partial
 __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const FinArgs *fin,
                                                                   int32_t *status) {
   const FinArgs f = fin[blockIdx.x];
-  finalize_body(f.ctl, f.part, f.nblk, f.area, f.n_cells, f.sync, f.dx, FIN_DRIVEN, 1);
+  finalize_body<kThreads>(f.ctl, f.part, f.nblk, f.area, f.n_cells, f.sync, f.dx, FIN_DRIVEN, 1);
   __syncthreads();
   if (threadIdx.x == 0) status[blockIdx.x] = (f.ctl->active == 1 ? 1 : 0) | (f.ctl->cur << 1) |
                                              (f.ctl->active == -1 ? 4 : 0);
@@ -227,7 +251,7 @@ int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
   hipLaunchKernelGGL(reduce_speed_kernel, dim3(nblk), dim3(kThreads), 0, s->stream, h, s->hu,
                      s->hv, s->pitch, s->ny, s->nx, scheme, s->part, rpb);
   RBS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinThreads), 0, s->stream, s->ctl, s->part,
                      nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx,
                      mode, 0);
   RBS_HIP(hipGetLastError());
@@ -236,7 +260,7 @@ int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
 
 int finalize_launch(rbs_state *s, int mode, int scheme) {
   const int nblk = scheme == RBS_SCHEME_HLLC ? hllc_num_blocks(s) : li_num_blocks(s);
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, s->stream, s->ctl, s->part,
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinThreads), 0, s->stream, s->ctl, s->part,
                      nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx, mode,
                      scheme == RBS_SCHEME_LI ? 1 : 0);
   RBS_HIP(hipGetLastError());

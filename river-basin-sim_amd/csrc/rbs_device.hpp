@@ -129,8 +129,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block reduction of (max f32, 3 x sum f64) over kThreads threads; result valid on
-// thread 0.  `scratch` must hold kThreads/64 * 4 doubles.
+// Block reduction of (max f32, 3 x sum f64) over NT threads; result valid on
+// thread 0.  `scratch` must hold NT/64 * 4 doubles.
+template <int NT = kThreads>
 __device__ __forceinline__ void block_reduce4(float &m, double &a, double &b, double &c,
                                               double *scratch) {
   m = wave_max(m);
@@ -138,7 +139,7 @@ __device__ __forceinline__ void block_reduce4(float &m, double &a, double &b, do
   b = wave_sum(b);
   c = wave_sum(c);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  constexpr int W = kThreads / 64;
+  constexpr int W = NT / 64;
   __syncthreads();
   if (lane == 0) {
     scratch[wid] = (double)m;

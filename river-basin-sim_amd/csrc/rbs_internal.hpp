@@ -175,12 +175,14 @@ int exner_launch(rbs_state *s, double porosity);
 int li_num_blocks(const rbs_state *s);
 int hllc_num_blocks(const rbs_state *s);
 int li_variant_count();
+int li_default_variant();
+void li_variant_shape(int v, int *bx, int *by, int *family);
 int ensure_partials(rbs_state *s, int nblk);
 
 int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
                  int64_t *steps0);
 int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream);
-int li_batch_launch(int variant, const LiArgs *members, const int *tile_member,
+int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
                     const int *tile_first, int total, hipStream_t stream);
 size_t li_args_size();
 void li_fill_args(const rbs_state *s, int limit, LiArgs *out);

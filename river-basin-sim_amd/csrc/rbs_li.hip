@@ -58,6 +58,138 @@ struct LiArgs {
   int32_t flags;
 };
 
+// ---- per-cell arithmetic shared by both kernel families ----------------------------
+// Both families call these with the same operands, so their results are identical
+// bit for bit (the build disables FMA contraction).
+
+// limit_outflow (local_inertial.py:197-239): the donor factor of one cell from its
+// four updated faces (left, right, top, bottom) and its depth.
+__device__ __forceinline__ float donor_beta(float ql, float qr, float qt, float qb, float h,
+                                            float dtdx) {
+  float q_out = 0.0f;
+  if (ql < 0.0f) q_out = q_out - ql;
+  if (qr > 0.0f) q_out = q_out + qr;
+  if (qt < 0.0f) q_out = q_out - qt;
+  if (qb > 0.0f) q_out = q_out + qb;
+  const float out_depth = dtdx * q_out;
+  return out_depth > 0.0f ? fclamp(h / out_depth, 0.0f, 1.0f) : 1.0f;
+}
+
+// limit_qx / limit_qy (:242-263): each face scaled by its donor's factor.
+// bc = this cell, bl/br/bt/bb = left/right/top/bottom neighbours.
+__device__ __forceinline__ void limit_faces(float &qxL, float &qxR, float &qyT, float &qyB,
+                                            float bc, float bl, float br, float bt, float bb) {
+  if (qxL > 0.0f) qxL = qxL * bl;
+  else if (qxL < 0.0f) qxL = qxL * bc;
+  if (qxR > 0.0f) qxR = qxR * bc;
+  else if (qxR < 0.0f) qxR = qxR * br;
+  if (qyT > 0.0f) qyT = qyT * bt;
+  else if (qyT < 0.0f) qyT = qyT * bc;
+  if (qyB > 0.0f) qyB = qyB * bc;
+  else if (qyB < 0.0f) qyB = qyB * bb;
+}
+
+struct CellSums {
+  float m_h = 0.0f;  // h_max starts at 0 (local_inertial.py:527)
+  double vol = 0.0, inf = 0.0, edge = 0.0;
+};
+
+struct StepScalars {
+  float dt, dtdx, dxdt, rain, rain_scale, rain_cont;
+};
+
+__device__ __forceinline__ StepScalars step_scalars(const LiArgs &a, const DevCtl *ctl) {
+  StepScalars k;
+  const double dt64 = ctl->dt;
+  k.dt = (float)dt64;
+  k.dtdx = k.dt / (float)a.dx64;
+  k.dxdt = (float)(dt64 / a.dx64);  // boundaries.py:179 (f64 quotient)
+  k.rain = ctl->rain;
+  k.rain_scale = ctl->rain_scale;
+  k.rain_cont = (a.flags & LI_COMP) ? 0.0f : k.rain;
+  return k;
+}
+
+// update_h (:132-151), areal sources (sources.py:293-329, local_inertial.py:154-167),
+// capped infiltration (:170-194) and the open-edge drains in the reference's launch
+// order (boundaries.py:181-208) for cell (r, c) with its limited faces.  `comp`
+// (Kahan debt), `rain_v` and `infil_v` are that cell's values, read by the caller;
+// the updated debt is returned in `comp`.
+__device__ __forceinline__ float cell_update(int flags, int r, int c, int ny, int nx,
+                                             float h_old, float qxL, float qxR, float qyT,
+                                             float qyB, const StepScalars &k, float &comp,
+                                             float rain_v, float infil_v, CellSums &sum) {
+  const float net = qxL - qxR + qyT - qyB;
+  float h = h_old + k.dtdx * net + k.rain_cont * k.dt;
+  if (flags & LI_COMP) {
+    if (!(flags & LI_FIELD)) kahan_add(h, comp, k.rain * k.dt);  // add_uniform_rain_c
+    else kahan_add(h, comp, rain_v * k.rain_scale * k.dt);        // add_rain_field_c
+  } else if (flags & LI_FIELD) {
+    h = h + rain_v * k.rain_scale * k.dt;                         // apply_rain_field
+  }
+  if (flags & LI_INFIL) {
+    const float avail = h;
+    float inf = infil_v * k.dt;
+    if (inf > avail) inf = avail;
+    if (inf < 0.0f) inf = 0.0f;
+    h = avail - inf;
+    sum.inf += (double)avail - (double)h;
+  }
+  if ((flags & LI_OPEN_W) && c == 0 && nx > 1) {
+    const float av = h;
+    h = drain(av, fmx(-qxR, 0.0f), k.dxdt);
+    sum.edge += (double)av - (double)h;
+  }
+  if ((flags & LI_OPEN_E) && c == nx - 1 && nx > 1) {
+    const float av = h;
+    h = drain(av, fmx(qxL, 0.0f), k.dxdt);
+    sum.edge += (double)av - (double)h;
+  }
+  if ((flags & LI_OPEN_N) && r == 0 && ny > 1) {
+    const float av = h;
+    h = drain(av, fmx(-qyB, 0.0f), k.dxdt);
+    sum.edge += (double)av - (double)h;
+  }
+  if ((flags & LI_OPEN_S) && r == ny - 1 && ny > 1) {
+    const float av = h;
+    h = drain(av, fmx(qyT, 0.0f), k.dxdt);
+    sum.edge += (double)av - (double)h;
+  }
+  sum.m_h = fmx(sum.m_h, h);
+  sum.vol += (double)h;
+  return h;
+}
+
+// accumulate_qs_x / accumulate_qs_y (sediment.py:218-273) for one face: the MPM
+// capacity at the face's flow depth, Kahan-added (times dt) to the integral at `o`.
+__device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_b, float z_a,
+                                                float z_b, float n_a, float n_b, float d_a,
+                                                float d_b, float dt, float *integ, float *debt,
+                                                int64_t o) {
+  const float h_face = fmx(eta_a, eta_b) - fmx(z_a, z_b);
+  const float qs = face_capacity(q, h_face, h_face, 0.5f * (n_a + n_b), 0.5f * (d_a + d_b));
+  float si = integ[o], sc = debt[o];
+  kahan_add(si, sc, qs * dt);
+  integ[o] = si;
+  debt[o] = sc;
+}
+
+__device__ __forceinline__ void write_partials(const LiArgs &a, int tile, CellSums &sum,
+                                               double *s_red) {
+  block_reduce4(sum.m_h, sum.vol, sum.inf, sum.edge, s_red);
+  if (threadIdx.x == 0) {
+    a.part.speed_max[tile] = sum.m_h;
+    a.part.vol[tile] = sum.vol;
+    a.part.infil[tile] = sum.inf;
+    a.part.edge[tile] = sum.edge;
+  }
+}
+
+// ---- family 1: LDS tile ---------------------------------------------------------------
+// BY x BX cells per block; the halo'd inputs are staged in LDS and the reference's
+// launches become phases separated by barriers.  A tile needs eta/z/n two cells
+// out, the old faces and h one cell out; faces in the halo are recomputed
+// bit-identically by the neighbouring tile.
 template <int BX, int BY>
 __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   const DevCtl *ctl = a.ctl;
@@ -89,20 +221,15 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   const int64_t P = a.pitch;
   const int tid = threadIdx.x;
   const int flags = a.flags;
-
-  const double dt64 = ctl->dt;
-  const float dt = (float)dt64;
+  const StepScalars k = step_scalars(a, ctl);
   const float dx = (float)a.dx64;
-  const float dtdx = dt / dx;
-  const float rain = ctl->rain;
-  const float rain_scale = ctl->rain_scale;
 
   // ---- phase A: stage the halo'd tile in LDS (clamped reads stay in bounds) ----
 #pragma unroll
   for (int k0 = 0; k0 < EH * EW; k0 += kThreads) {
-    const int k = k0 + tid;
-    if (k < EH * EW) {
-      const int lr = k / EW, lc = k - (k / EW) * EW;
+    const int kk = k0 + tid;
+    if (kk < EH * EW) {
+      const int lr = kk / EW, lc = kk - (kk / EW) * EW;
       const int gi = clampi(y0 + lr - 2, 0, ny - 1), gj = clampi(x0 + lc - 2, 0, nx - 1);
       const int64_t o = (int64_t)gi * P + gj;
       const float hv = h_in[o], zv = a.z[o];
@@ -114,18 +241,18 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   }
 #pragma unroll
   for (int k0 = 0; k0 < RH * QXW; k0 += kThreads) {
-    const int k = k0 + tid;
-    if (k < RH * QXW) {
-      const int lr = k / QXW, lf = k - (k / QXW) * QXW;
+    const int kk = k0 + tid;
+    if (kk < RH * QXW) {
+      const int lr = kk / QXW, lf = kk - (kk / QXW) * QXW;
       const int gi = clampi(y0 + lr - 1, 0, ny - 1), gf = clampi(x0 + lf - 1, 0, nx);
       s_qx[lr][lf] = qx_in[(int64_t)gi * P + gf];
     }
   }
 #pragma unroll
   for (int k0 = 0; k0 < QYH * RW; k0 += kThreads) {
-    const int k = k0 + tid;
-    if (k < QYH * RW) {
-      const int lf = k / RW, lc = k - (k / RW) * RW;
+    const int kk = k0 + tid;
+    if (kk < QYH * RW) {
+      const int lf = kk / RW, lc = kk - (kk / RW) * RW;
       const int gf = clampi(y0 + lf - 1, 0, ny), gj = clampi(x0 + lc - 1, 0, nx - 1);
       s_qy[lf][lc] = qy_in[(int64_t)gf * P + gj];
     }
@@ -137,170 +264,237 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   // and the reference's "interior faces only" launch ranges (:599-612).
 #pragma unroll
   for (int k0 = 0; k0 < RH * QXW; k0 += kThreads) {
-    const int k = k0 + tid;
-    if (k < RH * QXW) {
-      const int lr = k / QXW, lf = k - (k / QXW) * QXW;
+    const int kk = k0 + tid;
+    if (kk < RH * QXW) {
+      const int lr = kk / QXW, lf = kk - (kk / QXW) * QXW;
       const int r = y0 + lr - 1, f = x0 + lf - 1;
       float q = 0.0f;
       if (r >= 0 && r < ny && f >= 1 && f <= nx - 1)
         q = li_face(s_qx[lr][lf], s_eta[lr + 1][lf], s_eta[lr + 1][lf + 1], s_z[lr + 1][lf],
-                    s_z[lr + 1][lf + 1], s_n[lr + 1][lf], s_n[lr + 1][lf + 1], dx, dt);
+                    s_z[lr + 1][lf + 1], s_n[lr + 1][lf], s_n[lr + 1][lf + 1], dx, k.dt);
       s_qx[lr][lf] = q;
     }
   }
 #pragma unroll
   for (int k0 = 0; k0 < QYH * RW; k0 += kThreads) {
-    const int k = k0 + tid;
-    if (k < QYH * RW) {
-      const int lf = k / RW, lc = k - (k / RW) * RW;
+    const int kk = k0 + tid;
+    if (kk < QYH * RW) {
+      const int lf = kk / RW, lc = kk - (kk / RW) * RW;
       const int f = y0 + lf - 1, c = x0 + lc - 1;
       float q = 0.0f;
       if (c >= 0 && c < nx && f >= 1 && f <= ny - 1)
         q = li_face(s_qy[lf][lc], s_eta[lf][lc + 1], s_eta[lf + 1][lc + 1], s_z[lf][lc + 1],
-                    s_z[lf + 1][lc + 1], s_n[lf][lc + 1], s_n[lf + 1][lc + 1], dx, dt);
+                    s_z[lf + 1][lc + 1], s_n[lf][lc + 1], s_n[lf + 1][lc + 1], dx, k.dt);
       s_qy[lf][lc] = q;
     }
   }
   __syncthreads();
 
-  // ---- phase C: donor outflow limiter beta of the ring-1 cells (:197-239) ---------
+  // ---- phase C: donor outflow limiter beta of the ring-1 cells --------------------
   const bool limit = flags & LI_LIMIT;
   if (limit) {
 #pragma unroll
     for (int k0 = 0; k0 < RH * RW; k0 += kThreads) {
-      const int k = k0 + tid;
-      if (k < RH * RW) {
-        const int lr = k / RW, lc = k - (k / RW) * RW;
-        float q_out = 0.0f;
-        const float ql = s_qx[lr][lc];
-        if (ql < 0.0f) q_out = q_out - ql;
-        const float qr = s_qx[lr][lc + 1];
-        if (qr > 0.0f) q_out = q_out + qr;
-        const float qt = s_qy[lr][lc];
-        if (qt < 0.0f) q_out = q_out - qt;
-        const float qb = s_qy[lr + 1][lc];
-        if (qb > 0.0f) q_out = q_out + qb;
-        const float out_depth = dtdx * q_out;
-        s_beta[lr][lc] = out_depth > 0.0f ? fclamp(s_h[lr][lc] / out_depth, 0.0f, 1.0f) : 1.0f;
+      const int kk = k0 + tid;
+      if (kk < RH * RW) {
+        const int lr = kk / RW, lc = kk - (kk / RW) * RW;
+        s_beta[lr][lc] = donor_beta(s_qx[lr][lc], s_qx[lr][lc + 1], s_qy[lr][lc],
+                                    s_qy[lr + 1][lc], s_h[lr][lc], k.dtdx);
       }
     }
     __syncthreads();
   }
 
   // ---- phase D: limited faces, continuity, sources, sinks, edge drains ------------
-  float m_h = 0.0f;  // h_max starts at 0 (local_inertial.py:527)
-  double s_vol = 0.0, s_inf = 0.0, s_edge = 0.0;
-  const float dxdt = (float)(dt64 / a.dx64);  // boundaries.py:179 (f64 quotient)
-  const float rain_cont = (flags & LI_COMP) ? 0.0f : rain;
+  CellSums sum;
 #pragma unroll
   for (int k0 = 0; k0 < BY * BX; k0 += kThreads) {
-    const int k = k0 + tid;
-    const int lr = k / BX, lc = k - (k / BX) * BX;
+    const int kk = k0 + tid;
+    const int lr = kk / BX, lc = kk - (kk / BX) * BX;
     const int r = y0 + lr, c = x0 + lc;
-    if (k < BY * BX && r < ny && c < nx) {
+    if (kk < BY * BX && r < ny && c < nx) {
       float qxL = s_qx[lr + 1][lc + 1], qxR = s_qx[lr + 1][lc + 2];
       float qyT = s_qy[lr + 1][lc + 1], qyB = s_qy[lr + 2][lc + 1];
-      if (limit) {  // limit_qx / limit_qy (:242-263): scale by the donor's beta
-        const float bc = s_beta[lr + 1][lc + 1];
-        if (qxL > 0.0f) qxL = qxL * s_beta[lr + 1][lc];
-        else if (qxL < 0.0f) qxL = qxL * bc;
-        if (qxR > 0.0f) qxR = qxR * bc;
-        else if (qxR < 0.0f) qxR = qxR * s_beta[lr + 1][lc + 2];
-        if (qyT > 0.0f) qyT = qyT * s_beta[lr][lc + 1];
-        else if (qyT < 0.0f) qyT = qyT * bc;
-        if (qyB > 0.0f) qyB = qyB * bc;
-        else if (qyB < 0.0f) qyB = qyB * s_beta[lr + 2][lc + 1];
-      }
+      if (limit)
+        limit_faces(qxL, qxR, qyT, qyB, s_beta[lr + 1][lc + 1], s_beta[lr + 1][lc],
+                    s_beta[lr + 1][lc + 2], s_beta[lr][lc + 1], s_beta[lr + 2][lc + 1]);
       const int64_t o = (int64_t)r * P + c;
-      if (flags & LI_SED) {  // accumulate_qs_x/y (sediment.py:218-273), own faces only
-        if (c >= 1) {
-          const float eta_max = fmx(s_eta[lr + 2][lc + 1], s_eta[lr + 2][lc + 2]);
-          const float z_bank = fmx(s_z[lr + 2][lc + 1], s_z[lr + 2][lc + 2]);
-          const float h_face = eta_max - z_bank;
-          const float qs = face_capacity(qxL, h_face, h_face,
-                                         0.5f * (s_n[lr + 2][lc + 1] + s_n[lr + 2][lc + 2]),
-                                         0.5f * (a.d50[o - 1] + a.d50[o]));
-          float si = a.qs_int_x[o], sc = a.qs_comp_x[o];
-          kahan_add(si, sc, qs * dt);
-          a.qs_int_x[o] = si;
-          a.qs_comp_x[o] = sc;
-        }
-        if (r >= 1) {
-          const float eta_max = fmx(s_eta[lr + 1][lc + 2], s_eta[lr + 2][lc + 2]);
-          const float z_bank = fmx(s_z[lr + 1][lc + 2], s_z[lr + 2][lc + 2]);
-          const float h_face = eta_max - z_bank;
-          const float qs = face_capacity(qyT, h_face, h_face,
-                                         0.5f * (s_n[lr + 1][lc + 2] + s_n[lr + 2][lc + 2]),
-                                         0.5f * (a.d50[o - P] + a.d50[o]));
-          float si = a.qs_int_y[o], sc = a.qs_comp_y[o];
-          kahan_add(si, sc, qs * dt);
-          a.qs_int_y[o] = si;
-          a.qs_comp_y[o] = sc;
-        }
+      if (flags & LI_SED) {  // own faces only: left (c >= 1) and top (r >= 1)
+        if (c >= 1)
+          accumulate_face(qxL, s_eta[lr + 2][lc + 1], s_eta[lr + 2][lc + 2], s_z[lr + 2][lc + 1],
+                          s_z[lr + 2][lc + 2], s_n[lr + 2][lc + 1], s_n[lr + 2][lc + 2],
+                          a.d50[o - 1], a.d50[o], k.dt, a.qs_int_x, a.qs_comp_x, o);
+        if (r >= 1)
+          accumulate_face(qyT, s_eta[lr + 1][lc + 2], s_eta[lr + 2][lc + 2], s_z[lr + 1][lc + 2],
+                          s_z[lr + 2][lc + 2], s_n[lr + 1][lc + 2], s_n[lr + 2][lc + 2],
+                          a.d50[o - P], a.d50[o], k.dt, a.qs_int_y, a.qs_comp_y, o);
       }
-      // update_h (:132-151)
-      const float net = qxL - qxR + qyT - qyB;
-      float h = s_h[lr + 1][lc + 1] + dtdx * net + rain_cont * dt;
-      if (flags & LI_COMP) {
-        float cp = a.h_comp[o];
-        if (!(flags & LI_FIELD)) kahan_add(h, cp, rain * dt);          // add_uniform_rain_c
-        else kahan_add(h, cp, a.rain[o] * rain_scale * dt);            // add_rain_field_c
-        a.h_comp[o] = cp;
-      } else if (flags & LI_FIELD) {
-        h = h + a.rain[o] * rain_scale * dt;                           // apply_rain_field
-      }
-      if (flags & LI_INFIL) {  // apply_infiltration (:170-194)
-        const float avail = h;
-        float inf = a.infil[o] * dt;
-        if (inf > avail) inf = avail;
-        if (inf < 0.0f) inf = 0.0f;
-        h = avail - inf;
-        s_inf += (double)avail - (double)h;
-      }
-      // open edges in the reference's launch order (boundaries.py:181-208)
-      if ((flags & LI_OPEN_W) && c == 0 && nx > 1) {
-        const float av = h;
-        h = drain(av, fmx(-qxR, 0.0f), dxdt);
-        s_edge += (double)av - (double)h;
-      }
-      if ((flags & LI_OPEN_E) && c == nx - 1 && nx > 1) {
-        const float av = h;
-        h = drain(av, fmx(qxL, 0.0f), dxdt);
-        s_edge += (double)av - (double)h;
-      }
-      if ((flags & LI_OPEN_N) && r == 0 && ny > 1) {
-        const float av = h;
-        h = drain(av, fmx(-qyB, 0.0f), dxdt);
-        s_edge += (double)av - (double)h;
-      }
-      if ((flags & LI_OPEN_S) && r == ny - 1 && ny > 1) {
-        const float av = h;
-        h = drain(av, fmx(qyT, 0.0f), dxdt);
-        s_edge += (double)av - (double)h;
-      }
+      float comp = (flags & LI_COMP) ? a.h_comp[o] : 0.0f;
+      const float rain_v = (flags & LI_FIELD) ? a.rain[o] : 0.0f;
+      const float infil_v = (flags & LI_INFIL) ? a.infil[o] : 0.0f;
+      const float h = cell_update(flags, r, c, ny, nx, s_h[lr + 1][lc + 1], qxL, qxR, qyT, qyB,
+                                  k, comp, rain_v, infil_v, sum);
+      if (flags & LI_COMP) a.h_comp[o] = comp;
       h_out[o] = h;
       qx_out[o] = qxL;
       if (c == nx - 1) qx_out[o + 1] = qxR;
       qy_out[o] = qyT;
       if (r == ny - 1) qy_out[o + P] = qyB;
-      m_h = fmx(m_h, h);
-      s_vol += (double)h;
     }
   }
+  write_partials(a, tile, sum, s_red);
+}
 
-  block_reduce4(m_h, s_vol, s_inf, s_edge, s_red);
-  if (tid == 0) {
-    a.part.speed_max[tile] = m_h;
-    a.part.vol[tile] = s_vol;
-    a.part.infil[tile] = s_inf;
-    a.part.edge[tile] = s_edge;
+// ---- family 2: register strips ----------------------------------------------------------
+// Each wave owns a strip of 64 columns (global columns x0-2 .. x0+61) and marches
+// down H rows, keeping a rolling window of rows in registers: x-neighbours come from
+// lane shuffles, y-neighbours from the previous iterations, so there is no LDS and
+// no barrier.  Row R's inputs are ingested at iteration R (prefetched one iteration
+// ahead); its x-faces and the y-faces above it are updated at once, the donor factor
+// of row R-1 follows, and row R-2 is finished.  The two edge lanes on each side only
+// feed their neighbours: lanes 2..61 write, 60 columns per wave.  A wave starts two
+// rows above its first row (those faces and factors are recomputed bit-identically
+// by the wave above).  The block's waves are stacked vertically.
+constexpr int kStripW = 60;
+constexpr int kStripWaves = kThreads / 64;
+
+__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
+
+template <int H, bool SED>
+__device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
+  const DevCtl *ctl = a.ctl;
+  if (ctl->active != 1) return;
+  const int cur = ctl->cur;
+  const float *__restrict__ h_in = a.h[cur];
+  float *__restrict__ h_out = a.h[cur ^ 1];
+  const float *__restrict__ qx_in = a.qx[cur];
+  float *__restrict__ qx_out = a.qx[cur ^ 1];
+  const float *__restrict__ qy_in = a.qy[cur];
+  float *__restrict__ qy_out = a.qy[cur ^ 1];
+  const float *__restrict__ zf = a.z;
+  const float *__restrict__ nf = a.n;
+  __shared__ double s_red[4 * (kThreads / 64)];
+
+  const int ny = a.ny, nx = a.nx;
+  const int64_t P = a.pitch;
+  const int flags = a.flags;
+  const bool limit = flags & LI_LIMIT;
+  const StepScalars k = step_scalars(a, ctl);
+  const float dx = (float)a.dx64;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int c = tx * kStripW - 2 + lane;
+  const int y0 = (ty * kStripWaves + wave) * H;
+  const int y_end = y0 + H < ny ? y0 + H : ny;
+  const int cc = clampi(c, 0, nx - 1);  // clamped reads stay in bounds
+  const int cf = clampi(c, 0, nx);
+  const bool col_ok = c >= 0 && c < nx;
+  const bool xface_ok = c >= 1 && c <= nx - 1;  // interior x-face left of column c
+  const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
+
+  CellSums sum;
+  if (y0 < ny) {  // wave-uniform
+    // the window: row R-1 inputs, updated faces of rows R-1 and R-2, old depths of
+    // rows R-1 and R-2, donor factors of rows R-2 and R-3
+    float e1 = 0.0f, z1 = 0.0f, n1 = 0.0f;
+    float fx1 = 0.0f, fx2 = 0.0f, fy1 = 0.0f, fy2 = 0.0f;
+    float h1 = 0.0f, h2 = 0.0f, b2 = 1.0f, b3 = 1.0f;
+    float e2 = 0.0f, z2 = 0.0f, n2 = 0.0f, e3 = 0.0f, z3 = 0.0f, n3 = 0.0f;  // SED only
+    // prefetched row: h, z, n, old qx, old qy; and the next finished row's sources
+    float ph, pz, pn, pqx, pqy;
+    float pcomp = 0.0f, prain = 0.0f, pinfil = 0.0f;
+    auto ingest = [&](int R) {
+      const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;
+      ph = h_in[o + cc];
+      pz = zf[o + cc];
+      pn = nf[o + cc];
+      pqx = qx_in[o + cf];
+      pqy = qy_in[(int64_t)clampi(R, 0, ny) * P + cc];
+    };
+    auto sources = [&](int i) {
+      if (i >= y0 && i < y_end && own) {
+        const int64_t o = (int64_t)i * P + c;
+        if (flags & LI_COMP) pcomp = a.h_comp[o];
+        if (flags & LI_FIELD) prain = a.rain[o];
+        if (flags & LI_INFIL) pinfil = a.infil[o];
+      }
+    };
+    ingest(y0 - 2);
+    for (int it = 0; it < H + 4; ++it) {
+      const int R = y0 - 2 + it;
+      const float h0 = ph, z0 = pz, n0 = pn, qxo = pqx, qyo = pqy;
+      float comp = pcomp;
+      const float rain_v = prain, infil_v = pinfil;
+      if (it + 1 < H + 4) ingest(R + 1);
+      sources(R - 1);  // the row the next iteration finishes
+
+      // x-faces of row R (left face of each column) and y-faces above row R
+      const float e0 = h0 + z0;  // compute_eta (:61-69)
+      const float el = from_left(e0), zl = from_left(z0), nl = from_left(n0);
+      float fx0 = 0.0f, fy0 = 0.0f;
+      if (xface_ok && R >= 0 && R < ny) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, k.dt);
+      if (col_ok && R >= 1 && R <= ny - 1) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, k.dt);
+
+      // donor factor of row R-1
+      const float fx1r = from_right(fx1);
+      const float b1 = limit ? donor_beta(fx1, fx1r, fy1, fy0, h1, k.dtdx) : 1.0f;
+
+      // finish row i = R-2
+      const int i = R - 2;
+      const float fx2r = from_right(fx2);
+      const float b2l = from_left(b2), b2r = from_right(b2);
+      float e2l = 0.0f, z2l = 0.0f, n2l = 0.0f;
+      if (SED) {
+        e2l = from_left(e2);
+        z2l = from_left(z2);
+        n2l = from_left(n2);
+      }
+      if (i >= y0 && i < y_end && own) {
+        float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
+        if (limit) limit_faces(qxL, qxR, qyT, qyB, b2, b2l, b2r, b3, b1);
+        const int64_t o = (int64_t)i * P + c;
+        if (SED && (flags & LI_SED)) {
+          if (c >= 1)
+            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, a.d50[o - 1], a.d50[o], k.dt,
+                            a.qs_int_x, a.qs_comp_x, o);
+          if (i >= 1)
+            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, a.d50[o - P], a.d50[o], k.dt,
+                            a.qs_int_y, a.qs_comp_y, o);
+        }
+        const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
+                                    rain_v, infil_v, sum);
+        if (flags & LI_COMP) a.h_comp[o] = comp;
+        h_out[o] = h;
+        qx_out[o] = qxL;
+        if (c == nx - 1) qx_out[o + 1] = qxR;
+        qy_out[o] = qyT;
+        if (i == ny - 1) qy_out[o + P] = qyB;
+      }
+      // slide the window
+      if (SED) {
+        e3 = e2; z3 = z2; n3 = n2;
+        e2 = e1; z2 = z1; n2 = n1;
+      }
+      e1 = e0; z1 = z0; n1 = n0;
+      fx2 = fx1; fx1 = fx0;
+      fy2 = fy1; fy1 = fy0;
+      h2 = h1; h1 = h0;
+      b3 = b2; b2 = b1;
+    }
   }
+  write_partials(a, tile, sum, s_red);
 }
 
 // One state: one block per tile.
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
   li_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
+}
+template <int H, bool SED>
+__global__ __launch_bounds__(kThreads) void li_strip_kernel(LiArgs a) {
+  li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
 // A batch of states in one launch: the flat tile space of all members, mapped back
@@ -311,67 +505,90 @@ __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__rest
                                                             const int *__restrict__ tile_first,
                                                             int total) {
   const int t = xcd_tile(blockIdx.x, total);
-  const int m = tile_member[t];
-  const LiArgs a = members[m];
-  li_tile<BX, BY>(a, t - tile_first[m]);
+  // the member index is block-uniform: its arguments are read with scalar loads
+  const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
+  li_tile<BX, BY>(members[m], t - tile_first[m]);
+}
+template <int H, bool SED>
+__global__ __launch_bounds__(kThreads) void li_strip_batch_kernel(
+    const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
+    const int *__restrict__ tile_first, int total) {
+  const int t = xcd_tile(blockIdx.x, total);
+  const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
+  li_strip<H, SED>(members[m], t - tile_first[m]);
 }
 
-// ---- tile variants (the autotuner picks one per grid shape) ----------------------
+// ---- variants (the autotuner picks one per grid shape) -------------------------------
+// bx x by is the block's tile of cells.  `kern_sed` / `batch_sed` are the forms used
+// when a state carries the bedload integral (the LDS family handles it at run time).
+using StepFn = void (*)(LiArgs);
+using BatchFn = void (*)(const LiArgs *, const int *, const int *, int);
 struct LiVariant {
   int bx, by;
-  void (*kern)(LiArgs);
-  void (*batch)(const LiArgs *, const int *, const int *, int);
+  StepFn kern, kern_sed;
+  BatchFn batch, batch_sed;
 };
-#define RBS_LI_VARIANT(X, Y) {X, Y, li_step_kernel<X, Y>, li_batch_kernel<X, Y>}
+#define RBS_LI_TILE(X, Y) \
+  {X, Y, li_step_kernel<X, Y>, li_step_kernel<X, Y>, li_batch_kernel<X, Y>, li_batch_kernel<X, Y>}
+#define RBS_LI_STRIP(H)                                                                  \
+  {kStripW, kStripWaves * (H), li_strip_kernel<H, false>, li_strip_kernel<H, true>,     \
+   li_strip_batch_kernel<H, false>, li_strip_batch_kernel<H, true>}
 static const LiVariant kLiVariants[] = {
-    RBS_LI_VARIANT(64, 16), RBS_LI_VARIANT(64, 8),   RBS_LI_VARIANT(64, 32),
-    RBS_LI_VARIANT(128, 8), RBS_LI_VARIANT(128, 16), RBS_LI_VARIANT(32, 32),
+    RBS_LI_TILE(64, 16), RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
+    RBS_LI_TILE(128, 8), RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
+    RBS_LI_STRIP(16),    RBS_LI_STRIP(32),     RBS_LI_STRIP(64),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
+constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
+
+static const LiVariant &variant_at(int v) {
+  if (v < 0 || v >= kNumLiVariants) v = kDefaultLiVariant;
+  return kLiVariants[v];
+}
 
 int li_variant_count() { return kNumLiVariants; }
+int li_default_variant() { return kDefaultLiVariant; }
+void li_variant_shape(int v, int *bx, int *by, int *family) {
+  const LiVariant &x = variant_at(v);
+  *bx = x.bx;
+  *by = x.by;
+  *family = x.kern == x.kern_sed ? 0 : 1;
+}
 
 size_t li_args_size() { return sizeof(LiArgs); }
 
-int li_batch_launch(int variant, const LiArgs *members, const int *tile_member,
+int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
                     const int *tile_first, int total, hipStream_t stream) {
-  if (variant < 0 || variant >= kNumLiVariants) variant = 0;
-  hipLaunchKernelGGL(kLiVariants[variant].batch, dim3(total), dim3(kThreads), 0, stream, members,
-                     tile_member, tile_first, total);
+  const LiVariant &v = variant_at(variant);
+  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(total), dim3(kThreads), 0, stream,
+                     members, tile_member, tile_first, total);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }
 
-static const LiVariant &pick(const rbs_state *s) {
-  int v = s->li_variant;
-  if (v < 0 || v >= kNumLiVariants) v = 0;
-  return kLiVariants[v];
-}
-
 int li_num_blocks(const rbs_state *s) {
-  const LiVariant &v = pick(s);
+  const LiVariant &v = variant_at(s->li_variant);
   const int tx = (s->nx + v.bx - 1) / v.bx, ty = (s->ny + v.by - 1) / v.by;
   return tx * ty;
 }
 
-void li_fill_args(const rbs_state *s, int limit, LiArgs *out);
-
 int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
-  const LiVariant &v = pick(s);
+  const LiVariant &v = variant_at(s->li_variant);
   LiArgs a{};
   li_fill_args(s, limit, &a);
   if (a.n_tiles > s->part_cap) {
     set_error("partial buffer too small (%d < %d)", s->part_cap, a.n_tiles);
     return RBS_E_STATE;
   }
-  hipLaunchKernelGGL(v.kern, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
+  hipLaunchKernelGGL((a.flags & LI_SED) ? v.kern_sed : v.kern, dim3(a.n_tiles), dim3(kThreads),
+                     0, s->stream, a);
   RBS_HIP(hipGetLastError());
   s->cur ^= 1;
   return RBS_OK;
 }
 
 void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
-  const LiVariant &v = pick(s);
+  const LiVariant &v = variant_at(s->li_variant);
   LiArgs a{};
   for (int k = 0; k < 2; ++k) {
     a.h[k] = s->h[k];

@@ -266,7 +266,7 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
     s->li_variant = v;
     nblk = std::max(nblk, li_num_blocks(s));
   }
-  s->li_variant = 0;
+  s->li_variant = li_default_variant();
   nblk = std::max(nblk, reduce_partials_needed(s));
   nblk = std::max(nblk, hllc_num_blocks(s));
   TRY(ensure_partials(s, nblk));
@@ -580,6 +580,20 @@ int rbs_time_li_kernel(rbs_state *s, int32_t reps, double dt, double rain, doubl
 int rbs_li_variant_count(int32_t *count) {
   if (!count) return set_error("null count"), RBS_E_ARG;
   *count = li_variant_count();
+  return RBS_OK;
+}
+
+int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *family,
+                        int32_t *is_default) {
+  if (!bx || !by || !family || !is_default) return set_error("null argument"), RBS_E_ARG;
+  if (variant < 0 || variant >= li_variant_count())
+    return set_error("LI variant %d is not in [0, %d)", variant, li_variant_count()), RBS_E_ARG;
+  int x, y, f;
+  li_variant_shape(variant, &x, &y, &f);
+  *bx = x;
+  *by = y;
+  *family = f;
+  *is_default = variant == li_default_variant() ? 1 : 0;
   return RBS_OK;
 }
 

@@ -287,16 +287,17 @@ def basin_request(rid: int, arrival: float, pool: InputPool, rng, *, n_min=512, 
                    kind="basin")
 
 
-def reach_request(rid: int, arrival: float, pool: InputPool, rng, *, end_time=3600.0) -> Request:
+def reach_request(rid: int, arrival: float, pool: InputPool, rng, *, end_time=3600.0,
+                  scale: float = 1.0) -> Request:
     """Alluvial reach, dx 2 m, length x width log-uniform between 1024x256 and
     4096x1024: a trapezoidal channel (8 m wide, 1.5 m deep) on slope 1e-3 with 2 m of
     fBm floodplain relief, an inflow hydrograph on the channel's upstream cells,
     Exner + MPM with d50 U[0.5, 4] mm, porosity 0.4, morphological factor 1, open
-    downstream edge."""
+    downstream edge.  ``scale`` shrinks both sides (small test cases)."""
     from rbs.processes.inflow import Inflow
 
-    f = np.exp(rng.uniform(0.0, np.log(4.0)))
-    ny, nx = int(round(1024 * f)), int(round(256 * f))
+    f = np.exp(rng.uniform(0.0, np.log(4.0))) * scale
+    ny, nx = max(int(round(1024 * f)), 16), max(int(round(256 * f)), 16)
     dx = 2.0
     yy = np.arange(ny)[:, None] * np.ones((1, nx))
     xx = np.ones((ny, 1)) * np.arange(nx)[None, :]
@@ -317,16 +318,20 @@ def reach_request(rid: int, arrival: float, pool: InputPool, rng, *, end_time=36
 
 def make_requests(count: int, *, rate_per_s: float = 4.0, seed: int = 20261206,
                   pool: InputPool | None = None, end_time: float = 3600.0,
-                  basin_max: int = 4096) -> list[Request]:
+                  basin_min: int = 512, basin_max: int = 4096,
+                  reach_scale: float = 1.0) -> list[Request]:
     """The served mix: 70 % basins, 30 % alluvial reaches, Poisson arrivals."""
+    if not 0 < basin_min <= basin_max:
+        raise ValueError(f"basin sides need 0 < min <= max, got {basin_min}, {basin_max}")
     pool = pool or InputPool(seed=seed)
     rng = np.random.default_rng(seed)
     arrivals = poisson_arrivals(count, rate_per_s, seed + 7)
     out = []
     for k in range(count):
         if rng.uniform() < 0.7:
-            out.append(basin_request(k, arrivals[k], pool, rng, n_max=basin_max,
-                                     end_time=end_time))
+            out.append(basin_request(k, arrivals[k], pool, rng, n_min=basin_min,
+                                     n_max=basin_max, end_time=end_time))
         else:
-            out.append(reach_request(k, arrivals[k], pool, rng, end_time=end_time))
+            out.append(reach_request(k, arrivals[k], pool, rng, end_time=end_time,
+                                     scale=reach_scale))
     return out

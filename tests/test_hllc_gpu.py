@@ -95,7 +95,7 @@ def test_open_and_stage_edges_rain_infiltration(rbs, orc):
         t += dt
     d = _cmp(g, o)
     rg, ro = led_g.record(g, t), led_o.record(o, t)
-    assert rg.outflow_cum == pytest.approx(ro.outflow_cum, rel=1e-6, abs=1e-6)
+    assert rg.outflow_cum == pytest.approx(ro["outflow_cum"], rel=1e-6, abs=1e-6)
     assert rg.rel_error < 1e-6
     print(f"\n[hllc edges] max|dU| {d} outflow {rg.outflow_cum:.4f} mass {rg.rel_error:.2e}")
 

@@ -17,14 +17,11 @@ def test_served_results_equal_solo_results():
     from rbs.serve import InputPool, Server, make_requests, solo
 
     pool = InputPool(n=512, seed=11)
-    reqs = make_requests(8, rate_per_s=40.0, seed=11, pool=pool, end_time=600.0, basin_max=300)
-    # shrink the reaches to test size
+    reqs = make_requests(8, rate_per_s=40.0, seed=11, pool=pool, end_time=600.0,
+                         basin_min=96, basin_max=300, reach_scale=0.125)
     for r in reqs:
         if r.kind == "reach":
-            r.bed = np.ascontiguousarray(r.bed[:160, :48])
-            nx = r.bed.shape[1]
-            r.inflows = [i for i in r.inflows if i.cell[1] < nx] or r.inflows[:1]
-            r.interval_s = 200.0
+            r.interval_s = 200.0  # several morphology stops inside 600 s
     assert {r.kind for r in reqs} == {"basin", "reach"}
     srv = Server(capacity=16, chunk=16, keep_results=True)
     stats = srv.serve(reqs)
@@ -42,7 +39,8 @@ def test_admission_refuses_what_cannot_fit():
     from rbs.serve import InputPool, Server, make_requests
 
     pool = InputPool(n=256, seed=3)
-    reqs = make_requests(3, rate_per_s=100.0, seed=3, pool=pool, end_time=60.0, basin_max=200)
+    reqs = make_requests(3, rate_per_s=100.0, seed=3, pool=pool, end_time=60.0,
+                         basin_min=64, basin_max=200)
     for r in reqs:
         r.kind = "basin"
         r.d50 = None

@@ -57,9 +57,14 @@ def test_transport_integral_and_exner_match_the_oracle(rbs, orc):
     orc.exner_update(qx, qy, lo, hi, 5.0, 0.4, dz, un)
     zo = np.zeros_like(z)
     orc.rebuild_bed(z, dz, zo)
-    assert np.array_equal(sed.dz_cum.numpy(), dz)
-    assert np.array_equal(sed.dz_unapplied.numpy(), un)
-    assert np.array_equal(g.z.numpy(), zo)
+    # the same float64 arithmetic in numpy, as a second witness
+    want = -np.float64(np.float32(1.0 / 0.6)) * ((qx[:, 1:].astype(np.float64) - qx[:, :-1])
+                                                 + (qy[1:, :].astype(np.float64) - qy[:-1, :])) \
+        / np.float64(np.float32(5.0))
+    np.testing.assert_array_equal(dz, np.clip(want, lo, hi))
+    np.testing.assert_array_equal(sed.dz_cum.numpy(), dz)
+    np.testing.assert_array_equal(sed.dz_unapplied.numpy(), un)
+    np.testing.assert_array_equal(g.z.numpy(), zo)
     assert np.all(sed.qs_int_x.numpy() == 0.0)  # integral cleared ...
     assert np.array_equal(sed.qs_comp_x.numpy(), comp_before)  # ... its debt kept
     rec = led.record(g, 300.0)
