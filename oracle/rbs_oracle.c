@@ -437,12 +437,16 @@ void orc_accumulate_transport(orc_state *s, double dt) {
 void orc_exner_update(int32_t ny, int32_t nx, const float *qs_int_x, const float *qs_int_y,
                       const float *dz_lo, const float *dz_hi, double dx,
                       double inv_one_minus_p, double *dz_cum, double *dz_unapplied) {
-  /* dx and 1/(1-p) reach the kernel as float32 parameters, widened inside. */
-  const double dx64 = (double)(float)dx;
-  const double iomp = (double)(float)inv_one_minus_p;
+  /* dx and 1/(1-p) reach the kernel as float32 parameters, widened inside.  The
+   * narrowing is held in float variables: GCC 11 dropped a (double)(float) cast on
+   * a const double captured by the OpenMP region below. */
+  const float dx_f = (float)dx;
+  const float iomp_f = (float)inv_one_minus_p;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ny; ++i)
     for (int j = 0; j < nx; ++j) {
+      const double dx64 = (double)dx_f;
+      const double iomp = (double)iomp_f;
       double div_x = (double)qs_int_x[IX(i, j + 1, nx + 1)] - (double)qs_int_x[IX(i, j, nx + 1)];
       double div_y = (double)qs_int_y[IX(i + 1, j, nx)] - (double)qs_int_y[IX(i, j, nx)];
       double want = dz_cum[IX(i, j, nx)] - iomp * (div_x + div_y) / dx64;

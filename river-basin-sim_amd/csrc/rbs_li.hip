@@ -358,6 +358,17 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
 constexpr int kStripW = 60;
 constexpr int kStripWaves = kThreads / 64;
 
+// Element `k` of the row at `base` (a wave-uniform pointer) through a 32-bit byte
+// offset, so the load or store pairs a scalar base with one lane register.
+template <typename T>
+__device__ __forceinline__ T &at(T *base, uint32_t k) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + k * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, uint32_t k) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + k * (uint32_t)sizeof(T));
+}
+
 __device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
 
@@ -383,13 +394,19 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const StepScalars k = step_scalars(a, ctl);
   const float dx = (float)a.dx64;
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index, and so every row index below, is wave-uniform: row offsets live
+  // in scalar registers and only the column offset is per lane
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
   const int c = tx * kStripW - 2 + lane;
   const int y0 = (ty * kStripWaves + wave) * H;
   const int y_end = y0 + H < ny ? y0 + H : ny;
-  const int cc = clampi(c, 0, nx - 1);  // clamped reads stay in bounds
-  const int cf = clampi(c, 0, nx);
+  // clamped reads stay in bounds; unsigned offsets let a load pair a scalar row
+  // base with a 32-bit lane offset
+  const uint32_t cc = clampi(c, 0, nx - 1);
+  const uint32_t cf = clampi(c, 0, nx);
+  const uint32_t cu = (uint32_t)c;  // used only where 0 <= c < nx
   const bool col_ok = c >= 0 && c < nx;
   const bool xface_ok = c >= 1 && c <= nx - 1;  // interior x-face left of column c
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
@@ -406,19 +423,19 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     float ph, pz, pn, pqx, pqy;
     float pcomp = 0.0f, prain = 0.0f, pinfil = 0.0f;
     auto ingest = [&](int R) {
-      const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;
-      ph = h_in[o + cc];
-      pz = zf[o + cc];
-      pn = nf[o + cc];
-      pqx = qx_in[o + cf];
-      pqy = qy_in[(int64_t)clampi(R, 0, ny) * P + cc];
+      const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;  // uniform
+      ph = at(h_in + o, cc);
+      pz = at(zf + o, cc);
+      pn = at(nf + o, cc);
+      pqx = at(qx_in + o, cf);
+      pqy = at(qy_in + (int64_t)clampi(R, 0, ny) * P, cc);
     };
     auto sources = [&](int i) {
       if (i >= y0 && i < y_end && own) {
-        const int64_t o = (int64_t)i * P + c;
-        if (flags & LI_COMP) pcomp = a.h_comp[o];
-        if (flags & LI_FIELD) prain = a.rain[o];
-        if (flags & LI_INFIL) pinfil = a.infil[o];
+        const int64_t o = (int64_t)i * P;  // uniform
+        if (flags & LI_COMP) pcomp = at(a.h_comp + o, cu);
+        if (flags & LI_FIELD) prain = at(a.rain + o, cu);
+        if (flags & LI_INFIL) pinfil = at(a.infil + o, cu);
       }
     };
     ingest(y0 - 2);
@@ -454,7 +471,8 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
       if (i >= y0 && i < y_end && own) {
         float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
         if (limit) limit_faces(qxL, qxR, qyT, qyB, b2, b2l, b2r, b3, b1);
-        const int64_t o = (int64_t)i * P + c;
+        const int64_t row = (int64_t)i * P;  // uniform
+        const int64_t o = row + c;
         if (SED && (flags & LI_SED)) {
           if (c >= 1)
             accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, a.d50[o - 1], a.d50[o], k.dt,
@@ -465,12 +483,12 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         }
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
-        if (flags & LI_COMP) a.h_comp[o] = comp;
-        h_out[o] = h;
-        qx_out[o] = qxL;
-        if (c == nx - 1) qx_out[o + 1] = qxR;
-        qy_out[o] = qyT;
-        if (i == ny - 1) qy_out[o + P] = qyB;
+        if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
+        at(h_out + row, cu) = h;
+        at(qx_out + row, cu) = qxL;
+        if (c == nx - 1) at(qx_out + row, cu + 1) = qxR;
+        at(qy_out + row, cu) = qyT;
+        if (i == ny - 1) at(qy_out + row + P, cu) = qyB;
       }
       // slide the window
       if (SED) {

@@ -108,3 +108,20 @@ def test_transport_integral_x_y_symmetry(orc):
         orc.li_step(a, 0.5)
         orc.li_step(b, 0.5)
     np.testing.assert_allclose(a.qs_int_x, b.qs_int_y.T, rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.parametrize("dx,porosity", [(5.0, 0.4), (0.3, 0.35)])
+def test_exner_takes_float32_parameters(orc, dx, porosity):
+    """sediment.py:384-392: dx and 1/(1-p) are wp.float32 kernel parameters widened
+    inside, so a unit divergence moves the bed by exactly
+    -f64(f32(1/(1-p))) / f64(f32(dx)), not by the float64 quotient."""
+    qx = np.zeros((1, 2), np.float32)
+    qx[0, 1] = 1.0
+    qy = np.zeros((2, 1), np.float32)
+    lo = np.full((1, 1), -np.inf, np.float32)
+    hi = np.full((1, 1), np.inf, np.float32)
+    dz, un = np.zeros((1, 1)), np.zeros((1, 1))
+    orc.exner_update(qx, qy, lo, hi, dx, porosity, dz, un)
+    want = -float(np.float32(1.0 / (1.0 - porosity))) / float(np.float32(dx))
+    assert dz[0, 0] == want
+    assert dz[0, 0] != -(1.0 / (1.0 - porosity)) / dx
