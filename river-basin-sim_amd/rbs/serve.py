@@ -81,8 +81,9 @@ class Result:
         return self.done_s - self.arrival_s
 
 
-def build_state(req: Request):
-    """The device state + run of one request."""
+def build_state(req: Request, tune_kernel: bool = True):
+    """The device state + run of one request.  ``tune_kernel`` picks the fastest LI
+    kernel for the request's shape (a request run alone); a batch sets its own."""
     from rbs.core.sediment import arm_sediment
     from rbs.core.state import State
     from rbs.processes.inflow import InflowInjector
@@ -105,6 +106,10 @@ def build_state(req: Request):
         arm_sediment(st, req.d50, req.porosity)
         morph = MorphologyProcess(st, req.interval_s)
         procs = [morph.as_slow_process()]
+    if tune_kernel:
+        from rbs.autotune import tune
+
+        tune(st)
     events = [req.rain_duration] if req.rain is not None else []
     for inf in req.inflows:
         events.extend(inf.breakpoints)
@@ -168,7 +173,7 @@ class Server:
             return True
         if self.resident + need > self.budget or len(self.live) >= self.capacity:
             return False
-        st, run, morph = build_state(req)
+        st, run, morph = build_state(req, tune_kernel=False)
         segs = list(run.sched.segments())
         slot = self.batch.add(run, segs[0].t_stop)
         self.live[slot] = _Live(req, st, run, morph, segs, 0, slot, now, need)
