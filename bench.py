@@ -116,6 +116,8 @@ def main() -> int:
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--rate", type=float, default=4.0, help="served arrivals per second")
+    ap.add_argument("--capacity-sample", type=int, default=120,
+                    help="serve: requests replayed as a burst and as plain launches")
     ap.add_argument("--sim-hours", type=float, default=1.0, help="simulated time per scenario")
     args = ap.parse_args()
 
@@ -293,7 +295,9 @@ def bench_copy(args) -> int:
 
 
 def bench_ensemble(args) -> int:
-    """Batched ensemble vs the same scenarios run one after another (plain launches)."""
+    """Batched ensemble vs the same scenarios run one after another: as plain
+    per-scenario launches (the reference's tick loop over compute_dt/step) and as
+    device-driven graph-replayed runs.  Every scenario runs its shape's tuned kernel."""
     from rbs.driver import DeviceRun
     from rbs.ensemble import Batch, ensemble_inputs, make_member_state
     from rbs.scheduler import MultiRateScheduler
@@ -312,7 +316,31 @@ def bench_ensemble(args) -> int:
                         rain_duration=end, rain_field_sum=float(m.rain.astype(np.float64).sum()))
         return run
 
-    # sequential plain launches: one scenario at a time, each device-driven
+    # plain per-scenario launches: the reference's tick loop (run.py:369-388) over the
+    # scheme seam -- compute_dt (one reduction + a device-to-host read), fill_span on
+    # the host, one step launch -- one scenario after another
+    from rbs.core import local_inertial as li
+
+    t_plain = 0.0
+    plain_steps = 0
+    keep_plain = {}
+    for k, m in enumerate(members):
+        st = make_member_state(m)
+        sched = MultiRateScheduler(end_time=end, output_every=end)
+        st.sync()
+        t0 = time.perf_counter()
+        for tick in sched.ticks(lambda: li.compute_dt(st, 0.7, 30.0)):
+            li.step(st, tick.dt, rain=0.0, rain_scale=1.0 if tick.t0 < end - 1e-6 else 0.0,
+                    t=tick.t0)
+            plain_steps += 1
+        st.sync()
+        t_plain += time.perf_counter() - t0
+        if k % 37 == 0:
+            keep_plain[k] = st.h.numpy()
+        st.close()
+    log(f"[ensemble] plain per-scenario launches: {t_plain:.2f} s, {plain_steps} steps")
+
+    # sequential device-driven runs: one scenario at a time, steps replayed from a graph
     t_seq = 0.0
     seq_steps = 0
     keep = {}
@@ -338,6 +366,7 @@ def bench_ensemble(args) -> int:
     for sl in slots:
         b.remove(sl)
     same = all(np.array_equal(runs[k].state.h.numpy(), h) for k, h in keep.items())
+    same = same and all(np.array_equal(keep_plain[k], h) for k, h in keep.items())
     bat_steps = sum(r.state.ledger().steps for r in runs)
     log(f"[ensemble] batched: {t_bat:.2f} s wall, {bat_steps} steps, identical={same}")
     n = len(members)
@@ -356,9 +385,11 @@ def bench_ensemble(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"li_ensemble_{n}", "sides": "log-uniform 256..2048",
                    "dx_m": 10.0, "simulated_s": end, "cells_total": cells},
-        "sequential_scenarios_per_h": round(3600.0 * n / t_seq, 1),
-        "speedup_vs_sequential": round(t_seq / t_bat, 3),
-        "bitwise_identical_to_sequential": bool(same),
+        "plain_launches_scenarios_per_h": round(3600.0 * n / t_plain, 1),
+        "speedup_vs_plain_launches": round(t_plain / t_bat, 3),
+        "sequential_device_driven_scenarios_per_h": round(3600.0 * n / t_seq, 1),
+        "speedup_vs_sequential_device_driven": round(t_seq / t_bat, 3),
+        "bitwise_identical_batched_sequential_plain": bool(same),
         "batched_cell_updates_per_s": round(
             sum(m.bed.size * r.state.ledger().steps for m, r in zip(members, runs)) / t_bat / 1e9, 3),
     }
@@ -378,6 +409,7 @@ def bench_serve(args) -> int:
     srv = Server(capacity=256, chunk=32)
     stats = srv.serve(reqs, log=log)
     log(f"[serve] {stats}")
+    cap = serve_capacity(reqs[: args.capacity_sample]) if args.capacity_sample > 0 else None
     kinds = {}
     for r in srv.results:
         kinds.setdefault(r.kind, []).append(r.time_to_result)
@@ -406,8 +438,39 @@ def bench_serve(args) -> int:
         "peak_resident_gb": round(stats["peak_resident_gb"], 2),
         "wall_s": round(stats["wall_s"], 1),
     }
+    if cap:
+        out["capacity"] = cap
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serve_capacity(reqs) -> dict:
+    """Capacity of the server against plain per-request launches on the same requests:
+    the server takes them as one burst (all arrive at once); plain runs them one after
+    another through the reference's tick loop.  Every served result must equal its
+    plain result bit for bit."""
+    import dataclasses
+
+    from rbs.serve import Server, plain
+
+    burst = [dataclasses.replace(r, arrival_s=0.0) for r in reqs]
+    srv = Server(capacity=256, chunk=32, keep_results=True)
+    st = srv.serve(burst, log=log)
+    served = {r.rid: r for r in srv.results}
+    t_plain, same = 0.0, True
+    for r in reqs:
+        res = plain(r)
+        t_plain += res.done_s  # a plain run starts at 0: done_s is its wall time
+        same = same and res.steps == served[r.rid].steps and np.array_equal(res.h, served[r.rid].h)
+    n = len(reqs)
+    log(f"[serve] capacity on {n} requests: served burst {st['wall_s']:.1f} s, "
+        f"plain {t_plain:.1f} s, identical={same}")
+    return {"requests": n, "served_burst_wall_s": round(st["wall_s"], 2),
+            "plain_launches_wall_s": round(t_plain, 2),
+            "served_scenarios_per_h": round(3600.0 * n / st["wall_s"], 1),
+            "plain_launches_scenarios_per_h": round(3600.0 * n / t_plain, 1),
+            "speedup_vs_plain_launches": round(t_plain / st["wall_s"], 3),
+            "bitwise_identical_to_plain": bool(same)}
 
 
 def valley(n: int, dx: float, seed: int):

@@ -121,6 +121,33 @@ def build_state(req: Request, tune_kernel: bool = True):
     return st, run, morph
 
 
+def plain(req: Request, keep_h: bool = True) -> Result:
+    """Run one request alone the way the reference's run loop does (run.py:369-416):
+    every tick asks ``compute_dt`` (a reduction and a device-to-host read), the host
+    picks the step with ``fill_span``, applies point sources, launches one step, and
+    runs slow processes at their activations."""
+    from rbs.core import local_inertial as li
+    from rbs.processes.inflow import InflowInjector
+    from rbs.scheduler import EPS_T
+
+    t0 = time.perf_counter()
+    st, run, morph = build_state(req)
+    inj = InflowInjector(req.inflows, st) if req.inflows else None
+    steps = 0
+    for tick in run.sched.ticks(lambda: li.compute_dt(st, 0.7, 30.0)):
+        if inj is not None:
+            inj.apply(st, tick.t0, tick.dt)
+        raining = req.rain is not None and tick.t0 < req.rain_duration - EPS_T
+        li.step(st, tick.dt, rain=0.0, rain_scale=1.0 if raining else 0.0, t=tick.t0)
+        steps += 1
+        for proc, elapsed in tick.due:
+            proc.advance(tick.t1, elapsed)
+    h = st.h.numpy() if keep_h else None
+    t1 = time.perf_counter()
+    st.close()
+    return Result(req.rid, req.kind, req.bed.size, 0.0, 0.0, t1 - t0, steps, h)
+
+
 def solo(req: Request, keep_h: bool = True) -> Result:
     """Run one request alone (device-driven, no batch) -- the reference result."""
     t0 = time.perf_counter()

@@ -48,3 +48,25 @@ def test_admission_refuses_what_cannot_fit():
     srv = Server(capacity=4, hbm_budget=reqs[0].footprint_bytes() - 1)
     stats = srv.serve(reqs)
     assert stats["rejected"] >= 1
+
+
+def test_plain_launches_equal_device_driven_solo():
+    """The reference-style tick loop (compute_dt, host fill_span, sources, one step per
+    tick, morphology at its activations) and the device-driven solo run agree bit for
+    bit, for a basin and for a reach with point sources and Exner stops."""
+    from rbs.serve import InputPool, make_requests, plain, solo
+
+    pool = InputPool(n=256, seed=5)
+    reqs = make_requests(6, rate_per_s=10.0, seed=5, pool=pool, end_time=400.0,
+                         basin_min=64, basin_max=160, reach_scale=0.1)
+    kinds = set()
+    for r in reqs:
+        if r.kind == "reach":
+            r.interval_s = 100.0
+        if r.kind in kinds:
+            continue
+        kinds.add(r.kind)
+        a, b = plain(r), solo(r)
+        assert a.steps == b.steps
+        assert np.array_equal(a.h, b.h), f"request {r.rid} ({r.kind}) differs"
+    assert kinds == {"basin", "reach"}
