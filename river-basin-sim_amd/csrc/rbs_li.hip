@@ -72,7 +72,10 @@ __device__ __forceinline__ float donor_beta(float ql, float qr, float qt, float 
   if (qt < 0.0f) q_out = q_out - qt;
   if (qb > 0.0f) q_out = q_out + qb;
   const float out_depth = dtdx * q_out;
-  return out_depth > 0.0f ? fclamp(h / out_depth, 0.0f, 1.0f) : 1.0f;
+  // h >= out_depth > 0 gives h / out_depth >= 1 exactly (rounding is monotone), so
+  // the clamp returns 1 and the division can be skipped; NaN still divides
+  const bool divide = out_depth > 0.0f && !(h >= out_depth);
+  return divide ? fclamp(h / out_depth, 0.0f, 1.0f) : 1.0f;
 }
 
 // limit_qx / limit_qy (:242-263): each face scaled by its donor's factor.
@@ -374,7 +377,7 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t k) {
 __device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
 
-template <int H, bool SED>
+template <int H, bool SED, bool FLAT>
 __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const DevCtl *ctl = a.ctl;
   if (ctl->active != 1) return;
@@ -459,9 +462,21 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
       // x-faces of row R (left face of each column) and y-faces above row R
       const float e0 = h0 + z0;  // compute_eta (:61-69)
       const float el = from_left(e0), zl = from_left(z0), nl = from_left(n0);
+      const bool fx_valid = xface_ok && R >= 0 && R < ny;
+      const bool fy_valid = col_ok && R >= 1 && R <= ny - 1;
       float fx0 = 0.0f, fy0 = 0.0f;
-      if (xface_ok && R >= 0 && R < ny) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, rdx, k.dt);
-      if (col_ok && R >= 1 && R <= ny - 1) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, rdx, k.dt);
+      if (FLAT) {  // both faces in one basic block
+        bool ex_x, ex_y;
+        fx0 = li_face_flat(qxo, el, e0, zl, z0, nl, n0, dx, rdx, k.dt, fx_valid, ex_x);
+        fy0 = li_face_flat(qyo, e1, e0, z1, z0, n1, n0, dx, rdx, k.dt, fy_valid, ex_y);
+        if (!__all(ex_x && ex_y)) {  // rare: a dividend outside Markstein's range
+          if (!ex_x) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, rdx, k.dt);
+          if (!ex_y) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, rdx, k.dt);
+        }
+      } else {
+        if (fx_valid) fx0 = li_face(qxo, el, e0, zl, z0, nl, n0, dx, rdx, k.dt);
+        if (fy_valid) fy0 = li_face(qyo, e1, e0, z1, z0, n1, n0, dx, rdx, k.dt);
+      }
 
       // donor factor of row R-1
       const float fx1r = from_right(fx1);
@@ -519,9 +534,9 @@ template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
   li_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
-template <int H, bool SED>
+template <int H, bool SED, bool FLAT>
 __global__ __launch_bounds__(kThreads) void li_strip_kernel(LiArgs a) {
-  li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
+  li_strip<H, SED, FLAT>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
 // A batch of states in one launch: the flat tile space of all members, mapped back
@@ -536,13 +551,13 @@ __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__rest
   const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
   li_tile<BX, BY>(members[m], t - tile_first[m]);
 }
-template <int H, bool SED>
+template <int H, bool SED, bool FLAT>
 __global__ __launch_bounds__(kThreads) void li_strip_batch_kernel(
     const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
     const int *__restrict__ tile_first, int total) {
   const int t = xcd_tile(blockIdx.x, total);
   const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
-  li_strip<H, SED>(members[m], t - tile_first[m]);
+  li_strip<H, SED, FLAT>(members[m], t - tile_first[m]);
 }
 
 // ---- variants (the autotuner picks one per grid shape) -------------------------------
@@ -552,18 +567,22 @@ using StepFn = void (*)(LiArgs);
 using BatchFn = void (*)(const LiArgs *, const int *, const int *, int);
 struct LiVariant {
   int bx, by;
+  int family;  // 0 LDS tile, 1 register strip, 2 register strip with flat faces
   StepFn kern, kern_sed;
   BatchFn batch, batch_sed;
 };
-#define RBS_LI_TILE(X, Y) \
-  {X, Y, li_step_kernel<X, Y>, li_step_kernel<X, Y>, li_batch_kernel<X, Y>, li_batch_kernel<X, Y>}
-#define RBS_LI_STRIP(H)                                                                  \
-  {kStripW, kStripWaves * (H), li_strip_kernel<H, false>, li_strip_kernel<H, true>,     \
-   li_strip_batch_kernel<H, false>, li_strip_batch_kernel<H, true>}
+#define RBS_LI_TILE(X, Y)                                                                 \
+  {X, Y, 0, li_step_kernel<X, Y>, li_step_kernel<X, Y>, li_batch_kernel<X, Y>,            \
+   li_batch_kernel<X, Y>}
+#define RBS_LI_STRIP(H, F)                                                                    \
+  {kStripW, kStripWaves * (H), (F) ? 2 : 1, li_strip_kernel<H, false, F>,                   \
+   li_strip_kernel<H, true, F>,                                                               \
+   li_strip_batch_kernel<H, false, F>, li_strip_batch_kernel<H, true, F>}
 static const LiVariant kLiVariants[] = {
-    RBS_LI_TILE(64, 16), RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
-    RBS_LI_TILE(128, 8), RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
-    RBS_LI_STRIP(16),    RBS_LI_STRIP(32),     RBS_LI_STRIP(64),
+    RBS_LI_TILE(64, 16),      RBS_LI_TILE(64, 8),       RBS_LI_TILE(64, 32),
+    RBS_LI_TILE(128, 8),      RBS_LI_TILE(128, 16),     RBS_LI_TILE(32, 32),
+    RBS_LI_STRIP(16, false),  RBS_LI_STRIP(32, false),  RBS_LI_STRIP(64, false),
+    RBS_LI_STRIP(16, true),   RBS_LI_STRIP(32, true),   RBS_LI_STRIP(64, true),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
@@ -579,7 +598,7 @@ void li_variant_shape(int v, int *bx, int *by, int *family) {
   const LiVariant &x = variant_at(v);
   *bx = x.bx;
   *by = x.by;
-  *family = x.kern == x.kern_sed ? 0 : 1;
+  *family = x.family;
 }
 
 size_t li_args_size() { return sizeof(LiArgs); }
