@@ -262,7 +262,7 @@ def variant_sweep(st, dt: float, reps: int, cells: int, keep: int) -> list:
         _lib.call("rbs_time_li_kernel", st._handle, reps, dt, 0.0, 1.0, ctypes.byref(avg))
         us = avg.value * 1e3
         gbs = BYTES_PER_CELL * cells / (us * 1e-6) / 1e9
-        fam_name = ("lds_tile", "strip", "strip_flat")[fam.value]
+        fam_name = ("lds_tile", "strip", "strip2")[fam.value]
         rows.append({"variant": v, "family": fam_name, "tile": f"{bx.value}x{by.value}",
                      "default": bool(dflt.value), "kernel_us": round(us, 2),
                      "frac": round(gbs / HBM_PEAK_GBS, 4)})

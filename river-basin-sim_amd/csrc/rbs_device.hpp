@@ -103,31 +103,6 @@ __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, floa
   return num / den;
 }
 
-// li_face without branches, for kernels that evaluate several faces side by side:
-// a dry face (or one the caller marks invalid) returns 0 as li_face does, but its
-// lane still runs the arithmetic on a harmless depth, so independent faces share one
-// basic block and their long dependency chains interleave.  The dx division uses
-// Markstein's correction; `exact` is cleared when the dividend is zero, tiny or huge,
-// and the caller then recomputes that face with li_face.  A wet, exact face is
-// bit-identical to li_face.
-__device__ __forceinline__ float li_face_flat(float q, float eta_a, float eta_b, float z_a,
-                                              float z_b, float n_a, float n_b, float dx,
-                                              float rdx, float dt, bool valid, bool &exact) {
-  const float g = kG;
-  const float h_flow = fmx(eta_a, eta_b) - fmx(z_a, z_b);
-  const bool wet = valid && !(h_flow < kHDry);
-  const float hf = wet ? h_flow : 1.0f;
-  const float n_face = 0.5f * (n_a + n_b);
-  const float d = eta_b - eta_a;
-  const float q0 = d * rdx;
-  const float slope = __builtin_fmaf(__builtin_fmaf(-dx, q0, d), rdx, q0);
-  const float m = fabsf(d);
-  exact = !wet || (m >= 0x1p-100f && m <= 0x1p+100f);
-  const float num = q - g * hf * dt * slope;
-  const float den = manning_den(q, hf, n_face, g, dt);
-  return wet ? num / den : 0.0f;
-}
-
 // boundaries.py:94-106 _drain
 __device__ __forceinline__ float drain(float h_edge, float q_out, float dxdt) {
   float removed = dxdt * q_out;

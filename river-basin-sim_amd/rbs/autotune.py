@@ -29,7 +29,7 @@ def variant_table() -> list[dict]:
         bx, by, fam, dflt = (ctypes.c_int32() for _ in range(4))
         _lib.call("rbs_li_variant_info", v, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(fam),
                   ctypes.byref(dflt))
-        out.append({"variant": v, "family": ("lds_tile", "strip", "strip_flat")[fam.value],
+        out.append({"variant": v, "family": ("lds_tile", "strip", "strip2")[fam.value],
                     "tile": (bx.value, by.value), "default": bool(dflt.value)})
     return out
 
