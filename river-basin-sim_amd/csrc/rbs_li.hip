@@ -519,223 +519,17 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   write_partials(a, tile, sum, s_red);
 }
 
-// ---- family 2: register strips, two columns per lane -------------------------------
-// As family 1, but lane l holds the column pair (x0-2+2l, x0-1+2l): a wave spans 128
-// columns and writes the middle 124, rows move as 8-byte pairs, and the shuffles,
-// the row loop and the window bookkeeping are shared by two cells.  Within a lane
-// column a is finished before column b, so its per-block sums differ from family 1's
-// only in the association of the float64 ledger partials.
-constexpr int kStrip2W = 124;
-
-__device__ __forceinline__ float2 at2(const float *base, uint32_t k) {
-  return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(base) + k * 4u);
-}
-__device__ __forceinline__ void put2(float *base, uint32_t k, float x, float y) {
-  *reinterpret_cast<float2 *>(reinterpret_cast<char *>(base) + k * 4u) = make_float2(x, y);
-}
-
-template <int H, bool SED>
-__device__ __forceinline__ void li_strip2(const LiArgs &a, const int tile) {
-  const DevCtl *ctl = a.ctl;
-  if (ctl->active != 1) return;
-  const int cur = ctl->cur;
-  const float *__restrict__ h_in = a.h[cur];
-  float *__restrict__ h_out = a.h[cur ^ 1];
-  const float *__restrict__ qx_in = a.qx[cur];
-  float *__restrict__ qx_out = a.qx[cur ^ 1];
-  const float *__restrict__ qy_in = a.qy[cur];
-  float *__restrict__ qy_out = a.qy[cur ^ 1];
-  const float *__restrict__ zf = a.z;
-  const float *__restrict__ nf = a.n;
-  __shared__ double s_red[4 * (kThreads / 64)];
-
-  const int ny = a.ny, nx = a.nx;
-  const int64_t P = a.pitch;
-  const int flags = a.flags;
-  const bool limit = flags & LI_LIMIT;
-  const StepScalars k = step_scalars(a, ctl);
-  const float dx = (float)a.dx64;
-  const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
-
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-  const int ca = tx * kStrip2W - 2 + 2 * lane, cb = ca + 1;
-  const int y0 = (ty * kStripWaves + wave) * H;
-  const int y_end = y0 + H < ny ? y0 + H : ny;
-  // pair reads at an even, clamped column stay in bounds (the pitch is even and
-  // exceeds nx); the clamped values only reach out-of-domain columns
-  const uint32_t cp = clampi(ca, 0, (int)P - 2);
-  const uint32_t ua = (uint32_t)ca;  // used where column a is owned
-  const bool okA = ca >= 0 && ca < nx, okB = cb >= 0 && cb < nx;
-  const bool xfA = ca >= 1 && ca <= nx - 1, xfB = cb >= 1 && cb <= nx - 1;
-  const bool inner = lane >= 1 && lane <= 62;
-  const bool ownA = inner && okA, ownB = inner && okB;
-
-  CellSums sum;
-  if (y0 < ny) {  // wave-uniform
-    float e1a = 0.0f, e1b = 0.0f, z1a = 0.0f, z1b = 0.0f, n1a = 0.0f, n1b = 0.0f;
-    float fx1a = 0.0f, fx1b = 0.0f, fx2a = 0.0f, fx2b = 0.0f;
-    float fy1a = 0.0f, fy1b = 0.0f, fy2a = 0.0f, fy2b = 0.0f;
-    float h1a = 0.0f, h1b = 0.0f, h2a = 0.0f, h2b = 0.0f;
-    float b2a = 1.0f, b2b = 1.0f, b3a = 1.0f, b3b = 1.0f;
-    // SED only: inputs of rows R-2 and R-3
-    float e2a = 0.0f, e2b = 0.0f, z2a = 0.0f, z2b = 0.0f, n2a = 0.0f, n2b = 0.0f;
-    float e3a = 0.0f, e3b = 0.0f, z3a = 0.0f, z3b = 0.0f, n3a = 0.0f, n3b = 0.0f;
-    float2 ph, pz, pn, pqx, pqy;
-    float2 pcomp = make_float2(0.0f, 0.0f), prain = pcomp, pinfil = pcomp;
-    auto ingest = [&](int R) {
-      const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;  // uniform
-      ph = at2(h_in + o, cp);
-      pz = at2(zf + o, cp);
-      pn = at2(nf + o, cp);
-      pqx = at2(qx_in + o, cp);
-      pqy = at2(qy_in + (int64_t)clampi(R, 0, ny) * P, cp);
-    };
-    auto sources = [&](int i) {
-      if (i >= y0 && i < y_end) {
-        const int64_t o = (int64_t)i * P;
-        if (flags & LI_COMP) pcomp = at2(a.h_comp + o, cp);
-        if (flags & LI_FIELD) prain = at2(a.rain + o, cp);
-        if (flags & LI_INFIL) pinfil = at2(a.infil + o, cp);
-      }
-    };
-    // row y0-2 only feeds the y-faces above row y0-1
-    ingest(y0 - 2);
-    e1a = ph.x + pz.x;
-    e1b = ph.y + pz.y;
-    z1a = pz.x;
-    z1b = pz.y;
-    n1a = pn.x;
-    n1b = pn.y;
-    h1a = ph.x;
-    h1b = ph.y;
-    ingest(y0 - 1);
-    for (int it = 1; it < H + 4; ++it) {
-      const int R = y0 - 2 + it;
-      const float2 h0 = ph, z0 = pz, n0 = pn, qxo = pqx, qyo = pqy;
-      float2 comp = pcomp;
-      const float2 rain_v = prain, infil_v = pinfil;
-      if (it + 1 < H + 4) ingest(R + 1);
-      sources(R - 1);
-
-      // x-faces of row R and the y-faces above it, for both columns
-      const float e0a = h0.x + z0.x, e0b = h0.y + z0.y;  // compute_eta (:61-69)
-      const float elA = from_left(e0b), zlA = from_left(z0.y), nlA = from_left(n0.y);
-      const bool rowx = R >= 0 && R < ny, rowy = R >= 1 && R <= ny - 1;
-      float fx0a = 0.0f, fx0b = 0.0f, fy0a = 0.0f, fy0b = 0.0f;
-      if (xfA && rowx) fx0a = li_face(qxo.x, elA, e0a, zlA, z0.x, nlA, n0.x, dx, rdx, k.dt);
-      if (xfB && rowx) fx0b = li_face(qxo.y, e0a, e0b, z0.x, z0.y, n0.x, n0.y, dx, rdx, k.dt);
-      if (okA && rowy) fy0a = li_face(qyo.x, e1a, e0a, z1a, z0.x, n1a, n0.x, dx, rdx, k.dt);
-      if (okB && rowy) fy0b = li_face(qyo.y, e1b, e0b, z1b, z0.y, n1b, n0.y, dx, rdx, k.dt);
-
-      // donor factors of row R-1 (column b's right face is the next lane's fx a)
-      const float fx1a_r = from_right(fx1a);
-      float b1a = 1.0f, b1b = 1.0f;
-      if (limit) {
-        b1a = donor_beta(fx1a, fx1b, fy1a, fy0a, h1a, k.dtdx);
-        b1b = donor_beta(fx1b, fx1a_r, fy1b, fy0b, h1b, k.dtdx);
-      }
-
-      // finish row i = R-2
-      const int i = R - 2;
-      const float fx2a_r = from_right(fx2a);
-      const float b2b_l = from_left(b2b), b2a_r = from_right(b2a);
-      float e2b_l = 0.0f, z2b_l = 0.0f, n2b_l = 0.0f;
-      if (SED) {
-        e2b_l = from_left(e2b);
-        z2b_l = from_left(z2b);
-        n2b_l = from_left(n2b);
-      }
-      if (i >= y0 && i < y_end) {  // wave-uniform
-        float qxLa = fx2a, qxRa = fx2b, qyTa = fy2a, qyBa = fy1a;
-        float qxLb = fx2b, qxRb = fx2a_r, qyTb = fy2b, qyBb = fy1b;
-        if (limit) {
-          limit_faces(qxLa, qxRa, qyTa, qyBa, b2a, b2b_l, b2b, b3a, b1a);
-          limit_faces(qxLb, qxRb, qyTb, qyBb, b2b, b2a, b2a_r, b3b, b1b);
-        }
-        const int64_t row = (int64_t)i * P;  // uniform
-        if (SED && (flags & LI_SED)) {
-          if (ownA) {
-            const int64_t o = row + ca;
-            if (ca >= 1)
-              accumulate_face(qxLa, e2b_l, e2a, z2b_l, z2a, n2b_l, n2a, a.d50[o - 1], a.d50[o],
-                              k.dt, a.qs_int_x, a.qs_comp_x, o);
-            if (i >= 1)
-              accumulate_face(qyTa, e3a, e2a, z3a, z2a, n3a, n2a, a.d50[o - P], a.d50[o], k.dt,
-                              a.qs_int_y, a.qs_comp_y, o);
-          }
-          if (ownB) {
-            const int64_t o = row + cb;
-            accumulate_face(qxLb, e2a, e2b, z2a, z2b, n2a, n2b, a.d50[o - 1], a.d50[o], k.dt,
-                            a.qs_int_x, a.qs_comp_x, o);  // cb >= 1 always
-            if (i >= 1)
-              accumulate_face(qyTb, e3b, e2b, z3b, z2b, n3b, n2b, a.d50[o - P], a.d50[o], k.dt,
-                              a.qs_int_y, a.qs_comp_y, o);
-          }
-        }
-        float hA = 0.0f, hB = 0.0f;
-        if (ownA)
-          hA = cell_update(flags, i, ca, ny, nx, h2a, qxLa, qxRa, qyTa, qyBa, k, comp.x,
-                           rain_v.x, infil_v.x, sum);
-        if (ownB)
-          hB = cell_update(flags, i, cb, ny, nx, h2b, qxLb, qxRb, qyTb, qyBb, k, comp.y,
-                           rain_v.y, infil_v.y, sum);
-        if (ownA && ownB) {
-          put2(h_out + row, ua, hA, hB);
-          put2(qx_out + row, ua, qxLa, qxLb);
-          put2(qy_out + row, ua, qyTa, qyTb);
-          if (flags & LI_COMP) put2(a.h_comp + row, ua, comp.x, comp.y);
-        } else {
-          if (ownA) {
-            at(h_out + row, ua) = hA;
-            at(qx_out + row, ua) = qxLa;
-            at(qy_out + row, ua) = qyTa;
-            if (flags & LI_COMP) at(a.h_comp + row, ua) = comp.x;
-          }
-          if (ownB) {
-            at(h_out + row, ua + 1) = hB;
-            at(qx_out + row, ua + 1) = qxLb;
-            at(qy_out + row, ua + 1) = qyTb;
-            if (flags & LI_COMP) at(a.h_comp + row, ua + 1) = comp.y;
-          }
-        }
-        if (ownA && ca == nx - 1) at(qx_out + row, ua + 1) = qxRa;  // the east boundary face
-        if (ownB && cb == nx - 1) at(qx_out + row, ua + 2) = qxRb;
-        if (i == ny - 1) {  // the south boundary faces
-          if (ownA) at(qy_out + row + P, ua) = qyBa;
-          if (ownB) at(qy_out + row + P, ua + 1) = qyBb;
-        }
-      }
-      // slide the window
-      if (SED) {
-        e3a = e2a; e3b = e2b; z3a = z2a; z3b = z2b; n3a = n2a; n3b = n2b;
-        e2a = e1a; e2b = e1b; z2a = z1a; z2b = z1b; n2a = n1a; n2b = n1b;
-      }
-      e1a = e0a; e1b = e0b; z1a = z0.x; z1b = z0.y; n1a = n0.x; n1b = n0.y;
-      fx2a = fx1a; fx2b = fx1b; fx1a = fx0a; fx1b = fx0b;
-      fy2a = fy1a; fy2b = fy1b; fy1a = fy0a; fy1b = fy0b;
-      h2a = h1a; h2b = h1b; h1a = h0.x; h1b = h0.y;
-      b3a = b2a; b3b = b2b; b2a = b1a; b2b = b1b;
-    }
-  }
-  write_partials(a, tile, sum, s_red);
-}
-
 // One state: one block per tile.
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
   li_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
-template <int H, bool SED>
-__global__ __launch_bounds__(kThreads) void li_strip_kernel(LiArgs a) {
-  li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
-}
-
+// W: the waves per SIMD the register allocator aims for (the bedload form keeps
+// its extra window registers at 4)
 template <int H, bool SED, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W)))
-void li_strip2_kernel(LiArgs a) {
-  li_strip2<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 4 : W)))
+void li_strip_kernel(LiArgs a) {
+  li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
 // A batch of states in one launch: the flat tile space of all members, mapped back
@@ -750,23 +544,14 @@ __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__rest
   const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
   li_tile<BX, BY>(members[m], t - tile_first[m]);
 }
-template <int H, bool SED>
-__global__ __launch_bounds__(kThreads) void li_strip_batch_kernel(
+template <int H, bool SED, int W>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 4 : W)))
+void li_strip_batch_kernel(
     const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
     const int *__restrict__ tile_first, int total) {
   const int t = xcd_tile(blockIdx.x, total);
   const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
   li_strip<H, SED>(members[m], t - tile_first[m]);
-}
-
-template <int H, bool SED, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W)))
-void li_strip2_batch_kernel(
-    const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
-    const int *__restrict__ tile_first, int total) {
-  const int t = xcd_tile(blockIdx.x, total);
-  const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
-  li_strip2<H, SED>(members[m], t - tile_first[m]);
 }
 
 // ---- variants (the autotuner picks one per grid shape) -------------------------------
@@ -776,27 +561,21 @@ using StepFn = void (*)(LiArgs);
 using BatchFn = void (*)(const LiArgs *, const int *, const int *, int);
 struct LiVariant {
   int bx, by;
-  int family;  // 0 LDS tile, 1 register strip (a column per lane), 2 (two columns per lane)
+  int family;  // 0 LDS tile, 1 register strip
   StepFn kern, kern_sed;
   BatchFn batch, batch_sed;
 };
 #define RBS_LI_TILE(X, Y)                                                                 \
   {X, Y, 0, li_step_kernel<X, Y>, li_step_kernel<X, Y>, li_batch_kernel<X, Y>,            \
    li_batch_kernel<X, Y>}
-#define RBS_LI_STRIP(H)                                                                   \
-  {kStripW, kStripWaves * (H), 1, li_strip_kernel<H, false>, li_strip_kernel<H, true>,     \
-   li_strip_batch_kernel<H, false>, li_strip_batch_kernel<H, true>}
-// W: the waves per SIMD the register allocator aims for
-#define RBS_LI_STRIP2(H, W)                                                                   \
-  {kStrip2W, kStripWaves * (H), 2, li_strip2_kernel<H, false, W>,                            \
-   li_strip2_kernel<H, true, W>, li_strip2_batch_kernel<H, false, W>,                        \
-   li_strip2_batch_kernel<H, true, W>}
+#define RBS_LI_STRIP(H, W)                                                                    \
+  {kStripW, kStripWaves * (H), 1, li_strip_kernel<H, false, W>, li_strip_kernel<H, true, W>,  \
+   li_strip_batch_kernel<H, false, W>, li_strip_batch_kernel<H, true, W>}
 static const LiVariant kLiVariants[] = {
-    RBS_LI_TILE(64, 16), RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
-    RBS_LI_TILE(128, 8), RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
-    RBS_LI_STRIP(16),    RBS_LI_STRIP(32),     RBS_LI_STRIP(64),
-    RBS_LI_STRIP2(16, 3), RBS_LI_STRIP2(32, 3), RBS_LI_STRIP2(64, 3),
-    RBS_LI_STRIP2(16, 4), RBS_LI_STRIP2(32, 4),
+    RBS_LI_TILE(64, 16),  RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
+    RBS_LI_TILE(128, 8),  RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
+    RBS_LI_STRIP(16, 6),  RBS_LI_STRIP(32, 6),  RBS_LI_STRIP(64, 6),
+    RBS_LI_STRIP(16, 8),  RBS_LI_STRIP(32, 8),  RBS_LI_STRIP(64, 8),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
