@@ -521,6 +521,8 @@ def bench_hllc(args) -> int:
     value = n * n * taken / el / 1e9
     led = st.ledger()
     log(f"[hllc] {taken} steps in {el:.2f} s -> {value:.2f} Gcell/s, t={led.t:.1f}")
+    st.close()
+    lake = lake_at_rest(bed, dx)
     bytes_cell = 32
     out = {
         "metric": METRIC,
@@ -541,9 +543,32 @@ def bench_hllc(args) -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_cell * value / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_cell": bytes_cell},
+        "lake_at_rest": lake,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def lake_at_rest(bed, dx: float, side: int = 1024, steps: int = 1000) -> dict:
+    """Flat water at 10 m over a crop of the valley bed (wet and dry cells), closed
+    walls: the largest velocity after ``steps`` HLLC steps (hydrostatic balance)."""
+    from rbs.core import hllc
+    from rbs.core.state import State
+
+    n = bed.shape[0]
+    r0 = (n - side) // 2
+    z = np.ascontiguousarray(bed[r0:r0 + side, r0:r0 + side])
+    st = State.from_bed(z, dx, depth=np.maximum(10.0 - z, 0.0).astype(np.float32), manning=0.035)
+    for _ in range(steps):
+        hllc.step(st, hllc.compute_dt(st, 0.45, 30.0))
+    u, v = st.velocities_numpy()
+    worst = float(max(np.abs(u).max(), np.abs(v).max()))
+    wet = float((st.h.numpy() > 1e-3).mean())
+    st.close()
+    log(f"[hllc] lake at rest {side}x{side}, {steps} steps, wet fraction {wet:.2f}: "
+        f"max|u,v| = {worst:.2e} m/s")
+    return {"grid": f"{side}x{side}", "steps": steps, "wet_fraction": round(wet, 3),
+            "max_velocity_m_s": worst}
 
 
 if __name__ == "__main__":

@@ -575,7 +575,6 @@ static const LiVariant kLiVariants[] = {
     RBS_LI_TILE(64, 16),  RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
     RBS_LI_TILE(128, 8),  RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
     RBS_LI_STRIP(16, 6),  RBS_LI_STRIP(32, 6),  RBS_LI_STRIP(64, 6),
-    RBS_LI_STRIP(16, 8),  RBS_LI_STRIP(32, 8),  RBS_LI_STRIP(64, 8),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
