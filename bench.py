@@ -550,15 +550,17 @@ def bench_hllc(args) -> int:
 
 
 def lake_at_rest(bed, dx: float, side: int = 1024, steps: int = 1000) -> dict:
-    """Flat water at 10 m over a crop of the valley bed (wet and dry cells), closed
-    walls: the largest velocity after ``steps`` HLLC steps (hydrostatic balance)."""
+    """Flat water 10 m above the lowest bed point of a central crop of the valley,
+    closed walls: the largest velocity after ``steps`` HLLC steps (hydrostatic
+    balance over the channel banks and the floodplain relief)."""
     from rbs.core import hllc
     from rbs.core.state import State
 
     n = bed.shape[0]
     r0 = (n - side) // 2
     z = np.ascontiguousarray(bed[r0:r0 + side, r0:r0 + side])
-    st = State.from_bed(z, dx, depth=np.maximum(10.0 - z, 0.0).astype(np.float32), manning=0.035)
+    eta = float(z.min()) + 10.0
+    st = State.from_bed(z, dx, depth=np.maximum(eta - z, 0.0).astype(np.float32), manning=0.035)
     for _ in range(steps):
         hllc.step(st, hllc.compute_dt(st, 0.45, 30.0))
     u, v = st.velocities_numpy()

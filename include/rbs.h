@@ -227,6 +227,10 @@ int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *fami
 /* The LI kernel variant a batch launches for all its members (the default unless
  * set); takes effect at the next advance. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant);
+/* Host-side cost of membership changes so far: table rebuilds and graph captures
+ * (count and wall milliseconds) and graph replays launched. */
+int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
+                    int64_t *graph_launches);
 
 #ifdef __cplusplus
 }

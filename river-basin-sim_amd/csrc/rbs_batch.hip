@@ -9,6 +9,7 @@
 // from a captured HIP graph; adding or removing a member rebuilds the tables and
 // the graph between replays, which is what continuous batching needs.
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "rbs_device.hpp"
@@ -34,6 +35,9 @@ struct rbs_batch {
   std::vector<int> active;     // per slot: 1 running, 0 done
   hipGraphExec_t graph = nullptr;
   bool dirty = true;
+  // host-side cost of membership changes
+  int64_t rebuilds = 0, graph_launches = 0;
+  double rebuild_ms = 0.0;
 };
 
 static void drop_graph(rbs_batch *b) {
@@ -199,18 +203,28 @@ int rbs_batch_remove(rbs_batch *b, int32_t slot) {
 
 int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
   if (!b || steps < 1) return set_error("bad arguments"), RBS_E_ARG;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool rebuilt = false;
   if (b->dirty) {
     int rc = rebuild(b);
     if (rc) return rc;
+    rebuilt = true;
   }
   const int n = (int)b->order.size();
   if (n > 0) {
     if (!b->graph) {
       int rc = capture(b);
       if (rc) return rc;
+      rebuilt = true;
+    }
+    if (rebuilt) {
+      b->rebuilds += 1;
+      b->rebuild_ms +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     const int launches = (steps + b->chunk - 1) / b->chunk;
     for (int L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(b->graph, b->stream));
+    b->graph_launches += launches;
     RBS_HIP(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
                            b->stream));
     RBS_HIP(hipStreamSynchronize(b->stream));
@@ -237,6 +251,15 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active) {
   if (!b || !active || slot < 0 || slot >= b->capacity || !b->slot[slot])
     return set_error("no member in slot %d", slot), RBS_E_ARG;
   *active = b->active[slot];
+  return RBS_OK;
+}
+
+int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
+                    int64_t *graph_launches) {
+  if (!b || !rebuilds || !rebuild_ms || !graph_launches) return set_error("null argument"), RBS_E_ARG;
+  *rebuilds = b->rebuilds;
+  *rebuild_ms = b->rebuild_ms;
+  *graph_launches = b->graph_launches;
   return RBS_OK;
 }
 

@@ -61,6 +61,15 @@ class Batch:
         _lib.call("rbs_batch_member_active", self._handle, int(slot), ctypes.byref(a))
         return bool(a.value)
 
+    def stats(self) -> dict:
+        """Membership-change cost so far (table rebuilds + graph captures) and replays."""
+        r, g = ctypes.c_int64(), ctypes.c_int64()
+        ms = ctypes.c_double()
+        _lib.call("rbs_batch_stats", self._handle, ctypes.byref(r), ctypes.byref(ms),
+                  ctypes.byref(g))
+        return {"rebuilds": int(r.value), "rebuild_s": ms.value / 1e3,
+                "graph_launches": int(g.value)}
+
     def run_to_completion(self, chunk: int = 64) -> None:
         while self.advance(chunk) > 0:
             pass

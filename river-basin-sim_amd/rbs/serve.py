@@ -192,6 +192,10 @@ class Server:
         self.rejected: list[int] = []
         self.peak_live = 0
         self.peak_bytes = 0
+        # wall time by activity: admission (state build + upload), batched stepping
+        # (including table rebuilds), stops (slow processes, results), idle waiting
+        self.t_admit = self.t_advance = self.t_stops = 0.0
+        self.n_advance = 0
 
     def _admit(self, req: Request, now: float) -> bool:
         need = req.footprint_bytes()
@@ -245,13 +249,20 @@ class Server:
             while pending and pending[0][0] <= t:
                 waiting.append(heapq.heappop(pending)[2])
             still = []
+            ta = time.perf_counter()
             for req in waiting:  # FIFO admission against the HBM budget
                 if not self._admit(req, now()):
                     still.append(req)
             waiting = still
+            tb = time.perf_counter()
+            self.t_admit += tb - ta
             if self.live:
                 self.batch.advance(self.chunk)
+                tc = time.perf_counter()
                 self._service_stops(now)
+                self.t_advance += tc - tb
+                self.t_stops += time.perf_counter() - tc
+                self.n_advance += 1
             elif pending:
                 time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
             if log and now() - last_log > 10.0:
@@ -261,6 +272,7 @@ class Server:
                     f"resident={self.resident / 1e9:.1f} GB")
         wall = now()
         ttr = np.array([r.time_to_result for r in self.results]) if self.results else np.zeros(1)
+        bstats = self.batch.stats()
         self.batch.close()
         return {
             "completed": len(self.results),
@@ -271,6 +283,11 @@ class Server:
             "p99_s": float(np.percentile(ttr, 99)),
             "peak_live": self.peak_live,
             "peak_resident_gb": self.peak_bytes / 1e9,
+            "time_admit_s": self.t_admit,
+            "time_advance_s": self.t_advance,
+            "time_stops_s": self.t_stops,
+            "advances": self.n_advance,
+            **{f"batch_{k}": v for k, v in bstats.items()},
         }
 
 
