@@ -174,6 +174,11 @@ int rbs_inflow_apply(rbs_state *st, double t, double dt, double *volume_out);
 /* exner_update + rebuild_bed + refresh eta + clear the transport integral (its
  * Kahan debt kept).  Bounds are RBS_F_DZ_LO / RBS_F_DZ_HI (default -inf/+inf). */
 int rbs_exner_activate(rbs_state *st, double porosity);
+/* The totals MorphologyProcess and SedimentLedger read from the bed-change fields
+ * (morphology.py:286-307, massbalance.py:193-306), reduced on the device in a fixed
+ * order in float64: totals[0..4] = {sum dz_cum, sum |dz_cum|, min dz_cum, max dz_cum,
+ * sum dz_unapplied}. */
+int rbs_sed_totals(rbs_state *st, double *totals);
 
 /* ---- ledger and device-driven runs -------------------------------------------------- */
 /* MassLedger inputs: device volume, sink totals, clock (massbalance.py:109-135). */
@@ -204,6 +209,14 @@ int rbs_batch_remove(rbs_batch *b, int32_t slot);
  * members whose clock reached their t_stop idle.  n_active receives the number of
  * members still running. */
 int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active);
+/* The same advance split in two, so the host can prepare the next requests while the
+ * device steps: rbs_batch_launch enqueues the steps and returns; rbs_batch_poll sets
+ * *done once they finished (never blocks); rbs_batch_collect waits for them and
+ * updates member status.  While an advance is in flight, add / remove / set_variant
+ * return RBS_E_STATE. */
+int rbs_batch_launch(rbs_batch *b, int32_t steps);
+int rbs_batch_poll(rbs_batch *b, int32_t *done);
+int rbs_batch_collect(rbs_batch *b, int32_t *n_active);
 /* Per-member status: 1 = still running, 0 = reached t_stop. */
 int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active);
 

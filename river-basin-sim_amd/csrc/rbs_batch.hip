@@ -38,6 +38,10 @@ struct rbs_batch {
   // host-side cost of membership changes
   int64_t rebuilds = 0, graph_launches = 0;
   double rebuild_ms = 0.0;
+  // an advance in flight (launched, not yet collected)
+  hipEvent_t done_ev = nullptr;
+  bool in_flight = false;
+  int in_flight_n = 0;
 };
 
 static void drop_graph(rbs_batch *b) {
@@ -140,6 +144,8 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipStreamCreate"));
+  if (hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(hip_fail(hipErrorUnknown, "hipEventCreate"));
   if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
       hipMalloc(&b->d_fin, sizeof(FinArgs) * capacity) != hipSuccess ||
       hipMalloc(&b->d_status, sizeof(int32_t) * capacity) != hipSuccess ||
@@ -159,6 +165,7 @@ int rbs_batch_destroy(rbs_batch *b) {
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->h_status) (void)hipHostFree(b->h_status);
+  if (b->done_ev) (void)hipEventDestroy(b->done_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RBS_OK;
@@ -167,6 +174,7 @@ int rbs_batch_destroy(rbs_batch *b) {
 int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t_stop,
                   int32_t *slot) {
   if (!b || !st || !p || !slot) return set_error("null argument"), RBS_E_ARG;
+  if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   if (p->scheme != RBS_SCHEME_LI) return set_error("a batch advances local-inertial scenarios"), RBS_E_ARG;
   if (!p->limit) return set_error("batched steps always run the donor limiter"), RBS_E_ARG;
   for (int k = 0; k < b->capacity; ++k)
@@ -189,6 +197,7 @@ int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t
 int rbs_batch_remove(rbs_batch *b, int32_t slot) {
   if (!b || slot < 0 || slot >= b->capacity || !b->slot[slot])
     return set_error("no member in slot %d", slot), RBS_E_ARG;
+  if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   RBS_HIP(hipStreamSynchronize(b->stream));
   rbs_state *s = b->slot[slot];
   RBS_HIP(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->stream));
@@ -201,8 +210,9 @@ int rbs_batch_remove(rbs_batch *b, int32_t slot) {
   return RBS_OK;
 }
 
-int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
+int rbs_batch_launch(rbs_batch *b, int32_t steps) {
   if (!b || steps < 1) return set_error("bad arguments"), RBS_E_ARG;
+  if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   const auto t0 = std::chrono::steady_clock::now();
   bool rebuilt = false;
   if (b->dirty) {
@@ -211,25 +221,51 @@ int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
     rebuilt = true;
   }
   const int n = (int)b->order.size();
-  if (n > 0) {
-    if (!b->graph) {
-      int rc = capture(b);
-      if (rc) return rc;
-      rebuilt = true;
-    }
-    if (rebuilt) {
-      b->rebuilds += 1;
-      b->rebuild_ms +=
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    const int launches = (steps + b->chunk - 1) / b->chunk;
-    for (int L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(b->graph, b->stream));
-    b->graph_launches += launches;
-    RBS_HIP(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
-                           b->stream));
-    RBS_HIP(hipStreamSynchronize(b->stream));
+  if (n == 0) return RBS_OK;
+  if (!b->graph) {
+    int rc = capture(b);
+    if (rc) return rc;
+    rebuilt = true;
+  }
+  if (rebuilt) {
+    b->rebuilds += 1;
+    b->rebuild_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const int launches = (steps + b->chunk - 1) / b->chunk;
+  for (int L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(b->graph, b->stream));
+  b->graph_launches += launches;
+  RBS_HIP(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                         b->stream));
+  RBS_HIP(hipEventRecord(b->done_ev, b->stream));
+  b->in_flight = true;
+  b->in_flight_n = n;
+  return RBS_OK;
+}
+
+int rbs_batch_poll(rbs_batch *b, int32_t *done) {
+  if (!b || !done) return set_error("null argument"), RBS_E_ARG;
+  if (!b->in_flight) {
+    *done = 1;
+    return RBS_OK;
+  }
+  const hipError_t e = hipEventQuery(b->done_ev);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return RBS_OK;
+  }
+  RBS_HIP(e);
+  *done = 1;
+  return RBS_OK;
+}
+
+int rbs_batch_collect(rbs_batch *b, int32_t *n_active) {
+  if (!b) return set_error("null batch"), RBS_E_ARG;
+  if (b->in_flight) {
+    RBS_HIP(hipEventSynchronize(b->done_ev));
+    b->in_flight = false;
     bool changed = false;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < b->in_flight_n; ++i) {
       const int k = b->order[i];
       const int32_t w = b->h_status[i];
       b->slot[k]->cur = (w >> 1) & 1;
@@ -245,6 +281,12 @@ int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
   for (int k = 0; k < b->capacity; ++k) c += (b->slot[k] && b->active[k]) ? 1 : 0;
   if (n_active) *n_active = c;
   return RBS_OK;
+}
+
+int rbs_batch_advance(rbs_batch *b, int32_t steps, int32_t *n_active) {
+  int rc = rbs_batch_launch(b, steps);
+  if (rc) return rc;
+  return rbs_batch_collect(b, n_active);
 }
 
 int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active) {
@@ -265,6 +307,7 @@ int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
 
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant) {
   if (!b) return set_error("null batch"), RBS_E_ARG;
+  if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   if (variant < 0 || variant >= li_variant_count())
     return set_error("LI variant %d is not in [0, %d)", variant, li_variant_count()), RBS_E_ARG;
   if (variant != b->variant) {

@@ -135,6 +135,7 @@ struct rbs_state {
   float *d50 = nullptr, *qs_int_x = nullptr, *qs_comp_x = nullptr, *qs_int_y = nullptr,
         *qs_comp_y = nullptr, *z0 = nullptr, *dz_lo = nullptr, *dz_hi = nullptr;
   double *dz_cum = nullptr, *dz_unapplied = nullptr;
+  double *sed_part = nullptr;  // bed-change totals: 5 per block + the 5 results
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;

@@ -125,6 +125,73 @@ int exner_launch(rbs_state *s, double porosity) {
   return RBS_OK;
 }
 
+// Bed-change totals {sum dz_cum, sum |dz_cum|, min, max, sum dz_unapplied}.  Block b
+// folds rows [b*rpb, (b+1)*rpb) thread-strided, the block folds its threads in a
+// fixed tree, and one block folds the blocks in order: the same bits every call.
+constexpr int kSedThreads = 256;
+constexpr int kSedQ = 5;
+
+struct SedAcc {
+  double v[kSedQ];
+  __device__ SedAcc() : v{0.0, 0.0, INFINITY, -INFINITY, 0.0} {}
+  __device__ void fold(const double *o) {
+    v[0] += o[0];
+    v[1] += o[1];
+    v[2] = fmin(v[2], o[2]);
+    v[3] = fmax(v[3], o[3]);
+    v[4] += o[4];
+  }
+};
+
+__device__ __forceinline__ void sed_block_fold(SedAcc &acc, double *sh) {
+  const int t = threadIdx.x;
+  for (int q = 0; q < kSedQ; ++q) sh[q * kSedThreads + t] = acc.v[q];
+  __syncthreads();
+  for (int w = kSedThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      double o[kSedQ], m[kSedQ];
+      for (int q = 0; q < kSedQ; ++q) {
+        m[q] = sh[q * kSedThreads + t];
+        o[q] = sh[q * kSedThreads + t + w];
+      }
+      SedAcc x;
+      for (int q = 0; q < kSedQ; ++q) x.v[q] = m[q];
+      x.fold(o);
+      for (int q = 0; q < kSedQ; ++q) sh[q * kSedThreads + t] = x.v[q];
+    }
+    __syncthreads();
+  }
+  for (int q = 0; q < kSedQ; ++q) acc.v[q] = sh[q * kSedThreads];
+}
+
+__global__ __launch_bounds__(kSedThreads) void sed_totals_kernel(const double *dz, const double *un,
+                                                                 int64_t pitch, int ny, int nx,
+                                                                 int rpb, double *part) {
+  __shared__ double sh[kSedQ * kSedThreads];
+  SedAcc acc;
+  const int r0 = blockIdx.x * rpb;
+  for (int r = r0; r < r0 + rpb && r < ny; ++r)
+    for (int c = threadIdx.x; c < nx; c += kSedThreads) {
+      const int64_t o = (int64_t)r * pitch + c;
+      const double d = dz[o];
+      const double x[kSedQ] = {d, fabs(d), d, d, un[o]};
+      acc.fold(x);
+    }
+  sed_block_fold(acc, sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kSedQ; ++q) part[kSedQ * blockIdx.x + q] = acc.v[q];
+}
+
+__global__ __launch_bounds__(kSedThreads) void sed_totals_final(const double *part, int nblk,
+                                                                double *out) {
+  __shared__ double sh[kSedQ * kSedThreads];
+  SedAcc acc;
+  for (int b = threadIdx.x; b < nblk; b += kSedThreads) acc.fold(part + kSedQ * b);
+  sed_block_fold(acc, sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kSedQ; ++q) out[q] = acc.v[q];
+}
+
 }  // namespace rbs
 
 using namespace rbs;
@@ -198,6 +265,28 @@ int rbs_inflow_apply(rbs_state *s, double t, double dt, double *volume_out) {
                          hipMemcpyDeviceToHost, s->stream));
   RBS_HIP(hipStreamSynchronize(s->stream));
   s->speed_scheme = -1;  // h changed: the next compute_dt must reduce again
+  return RBS_OK;
+}
+
+int rbs_sed_totals(rbs_state *s, double *totals) {
+  if (!s || !totals) return set_error("null argument"), RBS_E_ARG;
+  if (!(s->armed & RBS_ARM_SED))
+    return set_error("bed-change totals need a state armed with RBS_ARM_SED"), RBS_E_STATE;
+  const int rpb = s->nx >= 4096 ? 1 : (4096 + s->nx - 1) / s->nx;
+  const int nblk = (s->ny + rpb - 1) / rpb;
+  if (!s->sed_part) {
+    const size_t bytes = sizeof(double) * ((size_t)kSedQ * nblk + kSedQ);
+    RBS_HIP(hipMalloc(&s->sed_part, bytes));
+    s->bytes += bytes;
+  }
+  double *out = s->sed_part + (size_t)kSedQ * nblk;
+  hipLaunchKernelGGL(sed_totals_kernel, dim3(nblk), dim3(kSedThreads), 0, s->stream, s->dz_cum,
+                     s->dz_unapplied, s->pitch, s->ny, s->nx, rpb, s->sed_part);
+  hipLaunchKernelGGL(sed_totals_final, dim3(1), dim3(kSedThreads), 0, s->stream, s->sed_part,
+                     nblk, out);
+  RBS_HIP(hipGetLastError());
+  RBS_HIP(hipMemcpyAsync(totals, out, sizeof(double) * kSedQ, hipMemcpyDeviceToHost, s->stream));
+  RBS_HIP(hipStreamSynchronize(s->stream));
   return RBS_OK;
 }
 

@@ -284,7 +284,7 @@ int rbs_state_destroy(rbs_state *s) {
   void *ptrs[] = {s->h[0], s->h[1], s->qx[0], s->qx[1], s->qy[0], s->qy[1], s->z, s->n,
                   s->h_comp, s->rain, s->infil, s->hu, s->hv, s->h1, s->hu1, s->hv1,
                   s->d50, s->qs_int_x, s->qs_comp_x, s->qs_int_y, s->qs_comp_y, s->z0,
-                  s->dz_lo, s->dz_hi, s->dz_cum, s->dz_unapplied, s->part.speed_max,
+                  s->dz_lo, s->dz_hi, s->dz_cum, s->dz_unapplied, s->sed_part, s->part.speed_max,
                   s->part.vol, s->part.infil, s->part.edge, s->ctl, s->d_sync,
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
                   s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol};

@@ -131,15 +131,11 @@ class SedimentLedger:
         return led
 
     def record(self, state: State, time: float) -> SedimentRecord:
-        import numpy as np
-
-        sed = state.sediment
+        tot = state.sediment.totals()  # float64 device reductions of dz_cum, dz_unapplied
         solid = self.cell_area * self.solid_fraction
-        dz = sed.dz_cum.numpy()
-        un = sed.dz_unapplied.numpy()
-        bed = float(dz.sum()) * solid
-        banked = float(un.sum()) * solid
-        gross = float(np.abs(dz).sum()) * solid
+        bed = tot["sum"] * solid
+        banked = tot["unapplied"] * solid
+        gross = tot["sum_abs"] * solid
         residual = -(bed + banked)
         self._peak_gross = max(self._peak_gross, gross)
         denom = max(abs(banked), self._peak_gross, 1e-12)

@@ -78,11 +78,21 @@ class SedimentState:
     def bed_change_numpy(self) -> np.ndarray:
         return self.dz_cum.numpy()
 
+    def totals(self) -> dict:
+        """Sum, sum of magnitudes, min and max of dz_cum and the sum of dz_unapplied,
+        reduced on the device (float64, fixed order): no field leaves the GPU."""
+        import ctypes
+
+        out = (ctypes.c_double * 5)()
+        _lib.call("rbs_sed_totals", self.state._handle, out)
+        return {"sum": out[0], "sum_abs": out[1], "min": out[2], "max": out[3],
+                "unapplied": out[4]}
+
     def solid_volume(self, cell_area: float) -> float:
-        return float(self.dz_cum.numpy().sum()) * cell_area * (1.0 - self.porosity)
+        return self.totals()["sum"] * cell_area * (1.0 - self.porosity)
 
     def banked_volume(self, cell_area: float) -> float:
-        return float(self.dz_unapplied.numpy().sum()) * cell_area * (1.0 - self.porosity)
+        return self.totals()["unapplied"] * cell_area * (1.0 - self.porosity)
 
     def summary(self) -> str:
         grain = (f"d50 = {1000.0 * self.d50_min:.2f}..{1000.0 * self.d50_max:.2f} mm"

@@ -56,6 +56,22 @@ class Batch:
         _lib.call("rbs_batch_advance", self._handle, int(steps), ctypes.byref(n))
         return int(n.value)
 
+    def launch(self, steps: int) -> None:
+        """Enqueue up to ``steps`` batched steps and return at once (see collect)."""
+        _lib.call("rbs_batch_launch", self._handle, int(steps))
+
+    def poll(self) -> bool:
+        """True once the launched steps have finished (never blocks)."""
+        d = ctypes.c_int32()
+        _lib.call("rbs_batch_poll", self._handle, ctypes.byref(d))
+        return bool(d.value)
+
+    def collect(self) -> int:
+        """Wait for the launched steps; returns the number of members still running."""
+        n = ctypes.c_int32()
+        _lib.call("rbs_batch_collect", self._handle, ctypes.byref(n))
+        return int(n.value)
+
     def active(self, slot: int) -> bool:
         a = ctypes.c_int32()
         _lib.call("rbs_batch_member_active", self._handle, int(slot), ctypes.byref(a))

@@ -95,16 +95,16 @@ class MorphologyProcess:
                 f"{self.interval_s} s: more than one interval has collapsed into a single "
                 "bed update")
         st, sed, g = self.state, self.sediment, self.state.grid
-        before = float(sed.dz_cum.numpy().sum())
+        before = sed.totals()["sum"]
         _lib.call("rbs_exner_activate", st._handle, float(sed.porosity))
         solid = g.cell_area * (1.0 - sed.porosity)
-        dz = sed.dz_cum.numpy()
-        cumulative = float(dz.sum())
+        tot = sed.totals()  # device reductions: the bed-change fields stay on the GPU
+        cumulative = tot["sum"]
         rec = BedChangeRecord(time=float(t), interval_s=float(dt_slow),
                               applied_m3=(cumulative - before) * solid,
                               cumulative_m3=cumulative * solid,
-                              banked_m3=float(sed.dz_unapplied.numpy().sum()) * solid,
-                              dz_min_m=float(dz.min()), dz_max_m=float(dz.max()))
+                              banked_m3=tot["unapplied"] * solid,
+                              dz_min_m=tot["min"], dz_max_m=tot["max"])
         self.records.append(rec)
         return rec
 

@@ -192,26 +192,34 @@ class Server:
         self.rejected: list[int] = []
         self.peak_live = 0
         self.peak_bytes = 0
-        # wall time by activity: admission (state build + upload), batched stepping
-        # (including table rebuilds), stops (slow processes, results), idle waiting
+        # wall time by activity: admission (state builds, overlapped with stepping),
+        # waiting for the batched steps, stops (slow processes, results)
         self.t_admit = self.t_advance = self.t_stops = 0.0
         self.n_advance = 0
+        self._n_prepared = 0
 
-    def _admit(self, req: Request, now: float) -> bool:
+    def _prepare(self, req: Request, now: float):
+        """Build the request's device state if it fits the HBM budget now.  Returns
+        the prepared member, ``False`` for a request that can never fit (refused), or
+        ``None`` when it must wait for memory or a free slot."""
         need = req.footprint_bytes()
         if need > self.budget:
             self.rejected.append(req.rid)  # can never fit: refuse, do not wait forever
-            return True
-        if self.resident + need > self.budget or len(self.live) >= self.capacity:
             return False
+        if self.resident + need > self.budget or len(self.live) + self._n_prepared >= self.capacity:
+            return None
         st, run, morph = build_state(req, tune_kernel=False)
         segs = list(run.sched.segments())
-        slot = self.batch.add(run, segs[0].t_stop)
-        self.live[slot] = _Live(req, st, run, morph, segs, 0, slot, now, need)
         self.resident += need
-        self.peak_live = max(self.peak_live, len(self.live))
         self.peak_bytes = max(self.peak_bytes, self.resident)
-        return True
+        self._n_prepared += 1
+        return _Live(req, st, run, morph, segs, 0, -1, now, need)
+
+    def _join(self, lv: "_Live") -> None:
+        lv.slot = self.batch.add(lv.run, lv.segments[0].t_stop)
+        self.live[lv.slot] = lv
+        self._n_prepared -= 1
+        self.peak_live = max(self.peak_live, len(self.live))
 
     def _service_stops(self, now_fn) -> None:
         """Members that reached a stop: run their slow processes, rejoin or finish."""
@@ -244,26 +252,42 @@ class Server:
         t_start = time.perf_counter()
         now = lambda: time.perf_counter() - t_start  # noqa: E731
         last_log = 0.0
-        while pending or waiting or self.live:
+        prepared: list[_Live] = []  # built states waiting to join the batch
+        while pending or waiting or self.live or prepared:
             t = now()
             while pending and pending[0][0] <= t:
                 waiting.append(heapq.heappop(pending)[2])
-            still = []
+            for lv in prepared:  # the batch is idle here: members join
+                self._join(lv)
+            prepared = []
+            launched = False
+            if self.live:
+                self.batch.launch(self.chunk)
+                launched = True
+                self.n_advance += 1
+            # FIFO admission: build the next requests' states while the device steps
+            # (one at a time while it is idle, so it gets work at once)
             ta = time.perf_counter()
-            for req in waiting:  # FIFO admission against the HBM budget
-                if not self._admit(req, now()):
-                    still.append(req)
-            waiting = still
+            while waiting:
+                if launched and self.batch.poll():
+                    break
+                lv = self._prepare(waiting[0], now())
+                if lv is None:
+                    break  # waits for memory or a slot
+                waiting.pop(0)
+                if lv is not False:
+                    prepared.append(lv)
+                if not launched:
+                    break
             tb = time.perf_counter()
             self.t_admit += tb - ta
-            if self.live:
-                self.batch.advance(self.chunk)
+            if launched:
+                self.batch.collect()
                 tc = time.perf_counter()
-                self._service_stops(now)
                 self.t_advance += tc - tb
+                self._service_stops(now)
                 self.t_stops += time.perf_counter() - tc
-                self.n_advance += 1
-            elif pending:
+            elif pending and not waiting and not prepared:
                 time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
             if log and now() - last_log > 10.0:
                 last_log = now()
@@ -284,7 +308,7 @@ class Server:
             "peak_live": self.peak_live,
             "peak_resident_gb": self.peak_bytes / 1e9,
             "time_admit_s": self.t_admit,
-            "time_advance_s": self.t_advance,
+            "time_wait_steps_s": self.t_advance,
             "time_stops_s": self.t_stops,
             "advances": self.n_advance,
             **{f"batch_{k}": v for k, v in bstats.items()},

@@ -70,6 +70,13 @@ def test_transport_integral_and_exner_match_the_oracle(rbs, orc):
     rec = led.record(g, 300.0)
     assert rec.rel_error < 1e-11  # massbalance.py:178 SEDIMENT_GATE
     assert rec.gross_volume > 0
+    # the device bed-change totals against numpy on the downloaded fields
+    tot = sed.totals()
+    assert tot["min"] == dz.min() and tot["max"] == dz.max()
+    assert tot["sum"] == pytest.approx(dz.sum(), rel=1e-12, abs=1e-18)
+    assert tot["sum_abs"] == pytest.approx(np.abs(dz).sum(), rel=1e-12)
+    assert tot["unapplied"] == pytest.approx(un.sum(), rel=1e-12, abs=1e-18)
+    assert tot == sed.totals()  # fixed-order reduction: the same bits again
 
 
 def test_point_sources_match_the_oracle_and_the_ledger(rbs, orc):
