@@ -22,10 +22,12 @@ struct rbs_batch {
   hipStream_t stream = nullptr;
   int variant = li_default_variant();
   int chunk = 16;
-  bool any_sed = false;  // a member carries the bedload integral
+  int plain_tiles = 0;   // tiles of members without the bedload integral (listed first)
   // device tables (rebuilt when membership changes)
   void *d_args = nullptr;      // LiArgs[capacity]
   FinArgs *d_fin = nullptr;    // [capacity]
+  InflowArgs *d_inflow = nullptr;  // [capacity]: members with point sources
+  int n_inflow = 0;
   int32_t *d_status = nullptr; // [capacity]
   int32_t *h_status = nullptr; // pinned mirror
   int *d_tile_member = nullptr, *d_tile_first = nullptr;
@@ -58,22 +60,24 @@ static int rebuild(rbs_batch *b) {
     if (b->slot[k] && b->active[k]) b->order.push_back(k);
   const int n = (int)b->order.size();
   b->total_tiles = 0;
-  b->any_sed = false;
+  b->plain_tiles = 0;
   b->dirty = false;
   if (n == 0) return RBS_OK;
   const size_t asz = li_args_size();
   std::vector<char> args(asz * n);
   std::vector<FinArgs> fin(n);
-  std::vector<int> first(n), member;
+  // two segments of the flat tile space, each launched with its own kernel form:
+  // members without the bedload integral first, then those with it; first[i] is
+  // member i's first tile within its segment
+  std::vector<int> first(n), member, member_sed;
   for (int i = 0; i < n; ++i) {
     rbs_state *s = b->slot[b->order[i]];
     s->li_variant = b->variant;
-    if (s->armed & RBS_ARM_SED) b->any_sed = true;
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
-    first[i] = b->total_tiles;
-    for (int t = 0; t < nt; ++t) member.push_back(i);
-    b->total_tiles += nt;
+    std::vector<int> &seg = (s->armed & RBS_ARM_SED) ? member_sed : member;
+    first[i] = (int)seg.size();
+    for (int t = 0; t < nt; ++t) seg.push_back(i);
     fin[i].ctl = s->ctl;
     fin[i].part = s->part;
     fin[i].sync = s->d_sync;
@@ -82,6 +86,18 @@ static int rebuild(rbs_batch *b) {
     fin[i].dx = s->dx;
     fin[i].nblk = nt;
   }
+  std::vector<InflowArgs> inflow;
+  for (int i = 0; i < n; ++i) {
+    rbs_state *s = b->slot[b->order[i]];
+    if (s->inflow.count > 0) {
+      inflow.emplace_back();
+      inflow_fill_args(s, &inflow.back());
+    }
+  }
+  b->n_inflow = (int)inflow.size();
+  b->plain_tiles = (int)member.size();
+  b->total_tiles = (int)(member.size() + member_sed.size());
+  member.insert(member.end(), member_sed.begin(), member_sed.end());
   if (b->total_tiles > b->tile_cap) {
     if (b->d_tile_member) RBS_HIP(hipFree(b->d_tile_member));
     b->d_tile_member = nullptr;
@@ -90,6 +106,9 @@ static int rebuild(rbs_batch *b) {
   }
   RBS_HIP(hipMemcpyAsync(b->d_args, args.data(), asz * n, hipMemcpyHostToDevice, b->stream));
   RBS_HIP(hipMemcpyAsync(b->d_fin, fin.data(), sizeof(FinArgs) * n, hipMemcpyHostToDevice, b->stream));
+  if (b->n_inflow > 0)
+    RBS_HIP(hipMemcpyAsync(b->d_inflow, inflow.data(), sizeof(InflowArgs) * b->n_inflow,
+                           hipMemcpyHostToDevice, b->stream));
   RBS_HIP(hipMemcpyAsync(b->d_tile_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
   RBS_HIP(hipMemcpyAsync(b->d_tile_member, member.data(), sizeof(int) * member.size(),
                          hipMemcpyHostToDevice, b->stream));
@@ -102,19 +121,17 @@ static int capture(rbs_batch *b) {
   RBS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBS_OK;
   for (int k = 0; k < b->chunk && rc == RBS_OK; ++k) {
-    // point sources of the members that have them (inflow.py:167-200), per member
-    for (int i = 0; i < n && rc == RBS_OK; ++i) {
-      rbs_state *s = b->slot[b->order[i]];
-      if (s->inflow.count > 0) {
-        hipStream_t keep = s->stream;
-        s->stream = b->stream;
-        rc = inflow_launch(s, true, 0.0, 0.0);
-        s->stream = keep;
-      }
-    }
+    // point sources of the members that have them (inflow.py:167-200), one launch
+    rc = inflow_batch_launch(b->d_inflow, b->n_inflow, b->stream);
     if (rc == RBS_OK)
-      rc = li_batch_launch(b->variant, b->any_sed, (const LiArgs *)b->d_args, b->d_tile_member,
-                           b->d_tile_first, b->total_tiles, b->stream);
+      rc = b->plain_tiles > 0
+               ? li_batch_launch(b->variant, false, (const LiArgs *)b->d_args, b->d_tile_member,
+                                 b->d_tile_first, b->plain_tiles, b->stream)
+               : RBS_OK;
+    if (rc == RBS_OK && b->total_tiles > b->plain_tiles)
+      rc = li_batch_launch(b->variant, true, (const LiArgs *)b->d_args,
+                           b->d_tile_member + b->plain_tiles, b->d_tile_first,
+                           b->total_tiles - b->plain_tiles, b->stream);
     if (rc == RBS_OK) rc = batch_finalize_launch(b->d_fin, n, b->d_status, b->stream);
   }
   hipGraph_t g = nullptr;
@@ -148,6 +165,7 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
     return fail(hip_fail(hipErrorUnknown, "hipEventCreate"));
   if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
       hipMalloc(&b->d_fin, sizeof(FinArgs) * capacity) != hipSuccess ||
+      hipMalloc(&b->d_inflow, sizeof(InflowArgs) * capacity) != hipSuccess ||
       hipMalloc(&b->d_status, sizeof(int32_t) * capacity) != hipSuccess ||
       hipMalloc(&b->d_tile_first, sizeof(int) * capacity) != hipSuccess)
     return fail(hip_fail(hipErrorOutOfMemory, "hipMalloc(batch tables)"));
@@ -161,7 +179,7 @@ int rbs_batch_destroy(rbs_batch *b) {
   if (!b) return RBS_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  void *p[] = {b->d_args, b->d_fin, b->d_status, b->d_tile_member, b->d_tile_first};
+  void *p[] = {b->d_args, b->d_fin, b->d_inflow, b->d_status, b->d_tile_member, b->d_tile_first};
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->h_status) (void)hipHostFree(b->h_status);

@@ -84,6 +84,19 @@ struct Partials {
 };
 
 // What the batched finalize needs of one member.
+// What one member's point-source step needs (device-driven or host-driven).
+struct InflowArgs {
+  float *h0, *h1;  // the LI ping-pong pair (the same buffer twice for HLLC)
+  int64_t pitch;
+  int count;
+  const int *rows, *cols, *off;
+  const double *kt, *kq;
+  float *comp;
+  double *vol;
+  DevCtl *ctl;
+  double area;
+};
+
 struct FinArgs {
   DevCtl *ctl;
   Partials part;
@@ -183,6 +196,8 @@ int ensure_partials(rbs_state *s, int nblk);
 int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
                  int64_t *steps0);
 int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream);
+void inflow_fill_args(const rbs_state *s, InflowArgs *out);
+int inflow_batch_launch(const InflowArgs *list, int n, hipStream_t stream);
 int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
                     const int *tile_first, int total, hipStream_t stream);
 size_t li_args_size();

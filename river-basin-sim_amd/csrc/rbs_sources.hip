@@ -29,54 +29,84 @@ __device__ double discharge_at(const double *kt, const double *kq, int n, double
   return kq[n - 1];
 }
 
-__global__ __launch_bounds__(kThreads) void inflow_kernel(float *h0, float *h1, int64_t pitch,
-                                                          int count,
-                                                          const int *rows, const int *cols,
-                                                          const int *off, const double *kt,
-                                                          const double *kq, float *comp,
-                                                          double *vol, DevCtl *ctl, int driven,
-                                                          double t_host, double dt_host,
-                                                          double area, double *out) {
+// One member's point sources for one step (the body of both launches below).
+__device__ void inflow_body(const InflowArgs &a, int driven, double t_host, double dt_host,
+                            double *out) {
+  const DevCtl *ctl = a.ctl;
   if (driven && ctl->active != 1) return;
   // the buffer holding the current depth: LI ping-pongs (parity on the device);
   // HLLC states pass the same buffer twice
-  float *h = ctl->cur ? h1 : h0;
+  float *h = ctl->cur ? a.h1 : a.h0;
   const double t = driven ? ctl->t : t_host;
   const double dt = driven ? ctl->dt : dt_host;
   const double t_mid = t + 0.5 * dt;  // inflow.py:180
-  for (int k = threadIdx.x; k < count; k += blockDim.x) {
-    const int n = off[k + 1] - off[k];
-    const double v = discharge_at(kt + off[k], kq + off[k], n, t_mid) * dt;
-    const float add = (float)(v / area);  // inflow.py:182
-    const int64_t o = (int64_t)rows[k] * pitch + cols[k];
-    float hh = h[o], cp = comp[k];
+  for (int k = threadIdx.x; k < a.count; k += blockDim.x) {
+    const int n = a.off[k + 1] - a.off[k];
+    const double v = discharge_at(a.kt + a.off[k], a.kq + a.off[k], n, t_mid) * dt;
+    const float add = (float)(v / a.area);  // inflow.py:182
+    const int64_t o = (int64_t)a.rows[k] * a.pitch + a.cols[k];
+    float hh = h[o], cp = a.comp[k];
     kahan_add(hh, cp, add);  // _inject_point_sources_c (:91-113)
     h[o] = hh;
-    comp[k] = cp;
-    vol[k] = (double)add;
+    a.comp[k] = cp;
+    a.vol[k] = (double)add;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
-    for (int k = 0; k < count; ++k) s += vol[k];
-    const double requested = s * area;  // inflow.py:200
+    for (int k = 0; k < a.count; ++k) s += a.vol[k];
+    const double requested = s * a.area;  // inflow.py:200
     if (driven)
-      kahan_add64(ctl->point_in, ctl->point_in_c, requested);
+      kahan_add64(a.ctl->point_in, a.ctl->point_in_c, requested);
     else
       *out = requested;
   }
 }
 
+__global__ __launch_bounds__(kThreads) void inflow_kernel(InflowArgs a, int driven, double t_host,
+                                                          double dt_host, double *out) {
+  inflow_body(a, driven, t_host, dt_host, out);
+}
+
+// A batch's point sources: one block per member that has any (device-driven).
+__global__ __launch_bounds__(kThreads) void inflow_batch_kernel(const InflowArgs *list) {
+  inflow_body(list[blockIdx.x], 1, 0.0, 0.0, nullptr);
+}
+
+void inflow_fill_args(const rbs_state *s, InflowArgs *out) {
+  const Inflows &in = s->inflow;
+  InflowArgs a{};
+  // an HLLC state keeps its depth in h[cur] without ping-pong
+  a.h0 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[0];
+  a.h1 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[1];
+  a.pitch = s->pitch;
+  a.count = in.count;
+  a.rows = in.d_rows;
+  a.cols = in.d_cols;
+  a.off = in.d_off;
+  a.kt = in.d_kt;
+  a.kq = in.d_kq;
+  a.comp = in.d_comp;
+  a.vol = in.d_vol;
+  a.ctl = s->ctl;
+  a.area = s->dx * s->dx;
+  *out = a;
+}
+
+int inflow_batch_launch(const InflowArgs *list, int n, hipStream_t stream) {
+  if (n < 1) return RBS_OK;
+  hipLaunchKernelGGL(inflow_batch_kernel, dim3(n), dim3(kThreads), 0, stream, list);
+  RBS_HIP(hipGetLastError());
+  return RBS_OK;
+}
+
 int inflow_launch(rbs_state *s, bool driven, double t, double dt) {
   Inflows &in = s->inflow;
   if (in.count < 1) return RBS_OK;
-  // an HLLC state keeps its depth in h[cur] without ping-pong
-  float *h0 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[0];
-  float *h1 = (s->armed & RBS_ARM_HLLC) ? s->h[s->cur] : s->h[1];
-  hipLaunchKernelGGL(inflow_kernel, dim3(1), dim3(kThreads), 0, s->stream, h0, h1,
-                     s->pitch, in.count, in.d_rows, in.d_cols, in.d_off, in.d_kt, in.d_kq,
-                     in.d_comp, in.d_vol, s->ctl, driven ? 1 : 0, t, dt, s->dx * s->dx,
-                     in.d_vol + in.count);
+  InflowArgs a;
+  inflow_fill_args(s, &a);
+  hipLaunchKernelGGL(inflow_kernel, dim3(1), dim3(kThreads), 0, s->stream, a, driven ? 1 : 0, t,
+                     dt, in.d_vol + in.count);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

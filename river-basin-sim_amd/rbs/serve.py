@@ -180,40 +180,46 @@ class Server:
     """Continuous-batching server over one GPU."""
 
     def __init__(self, *, capacity: int = 256, hbm_budget: int = int(0.85 * HBM_BYTES),
-                 chunk: int = 32, keep_results: bool = False):
+                 chunk: int = 32, keep_results: bool = False, prep_threads: int = 2):
         self.batch = Batch(capacity)
         self.capacity = capacity
         self.budget = hbm_budget
         self.chunk = chunk
         self.keep = keep_results
+        self.prep_threads = prep_threads
         self.resident = 0
         self.live: dict[int, _Live] = {}
         self.results: list[Result] = []
         self.rejected: list[int] = []
         self.peak_live = 0
         self.peak_bytes = 0
-        # wall time by activity: admission (state builds, overlapped with stepping),
-        # waiting for the batched steps, stops (slow processes, results)
+        # wall time by activity on the main thread: waiting for a first build with
+        # nothing to step, batched stepping, stops (slow processes, results)
         self.t_admit = self.t_advance = self.t_stops = 0.0
         self.n_advance = 0
         self._n_prepared = 0
 
-    def _prepare(self, req: Request, now: float):
-        """Build the request's device state if it fits the HBM budget now.  Returns
-        the prepared member, ``False`` for a request that can never fit (refused), or
-        ``None`` when it must wait for memory or a free slot."""
+    def _reserve(self, req: Request):
+        """Admission control: ``True`` when the request fits the HBM budget and a slot
+        now (its bytes are reserved), ``False`` for a request that can never fit
+        (refused), ``None`` when it must wait."""
         need = req.footprint_bytes()
         if need > self.budget:
             self.rejected.append(req.rid)  # can never fit: refuse, do not wait forever
             return False
         if self.resident + need > self.budget or len(self.live) + self._n_prepared >= self.capacity:
             return None
-        st, run, morph = build_state(req, tune_kernel=False)
-        segs = list(run.sched.segments())
         self.resident += need
         self.peak_bytes = max(self.peak_bytes, self.resident)
         self._n_prepared += 1
-        return _Live(req, st, run, morph, segs, 0, -1, now, need)
+        return True
+
+    @staticmethod
+    def _build(req: Request, now: float) -> "_Live":
+        """Build a request's device state (runs on the preparation thread)."""
+        st, run, morph = build_state(req, tune_kernel=False)
+        segs = list(run.sched.segments())
+        return _Live(req, st, run, morph, segs, 0, -1, now, req.footprint_bytes())
 
     def _join(self, lv: "_Live") -> None:
         lv.slot = self.batch.add(lv.run, lv.segments[0].t_stop)
@@ -252,48 +258,51 @@ class Server:
         t_start = time.perf_counter()
         now = lambda: time.perf_counter() - t_start  # noqa: E731
         last_log = 0.0
-        prepared: list[_Live] = []  # built states waiting to join the batch
-        while pending or waiting or self.live or prepared:
-            t = now()
-            while pending and pending[0][0] <= t:
-                waiting.append(heapq.heappop(pending)[2])
-            for lv in prepared:  # the batch is idle here: members join
-                self._join(lv)
-            prepared = []
-            launched = False
-            if self.live:
-                self.batch.launch(self.chunk)
-                launched = True
-                self.n_advance += 1
-            # FIFO admission: build the next requests' states while the device steps
-            # (one at a time while it is idle, so it gets work at once)
-            ta = time.perf_counter()
-            while waiting:
-                if launched and self.batch.poll():
-                    break
-                lv = self._prepare(waiting[0], now())
-                if lv is None:
-                    break  # waits for memory or a slot
-                waiting.pop(0)
-                if lv is not False:
-                    prepared.append(lv)
-                if not launched:
-                    break
-            tb = time.perf_counter()
-            self.t_admit += tb - ta
-            if launched:
-                self.batch.collect()
-                tc = time.perf_counter()
-                self.t_advance += tc - tb
-                self._service_stops(now)
-                self.t_stops += time.perf_counter() - tc
-            elif pending and not waiting and not prepared:
-                time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
-            if log and now() - last_log > 10.0:
-                last_log = now()
-                log(f"[serve] t={last_log:.0f}s done={len(self.results)} live={len(self.live)} "
-                    f"waiting={len(waiting)} pending={len(pending)} "
-                    f"resident={self.resident / 1e9:.1f} GB")
+        # States are built on a preparation thread while the main thread keeps the
+        # batch stepping; librbs handles are independent, and batch graph capture is
+        # thread-local, so the two never wait on each other.
+        from concurrent.futures import ThreadPoolExecutor
+
+        pool = ThreadPoolExecutor(max_workers=self.prep_threads)
+        building: list = []  # futures of _Live, in admission order
+        try:
+            while pending or waiting or self.live or building:
+                t = now()
+                while pending and pending[0][0] <= t:
+                    waiting.append(heapq.heappop(pending)[2])
+                while waiting:  # FIFO admission against the HBM budget and the slots
+                    ok = self._reserve(waiting[0])
+                    if ok is None:
+                        break
+                    req = waiting.pop(0)
+                    if ok:
+                        building.append(pool.submit(self._build, req, now()))
+                ready = [f for f in building if f.done()]
+                building = [f for f in building if not f.done()]
+                for f in ready:  # the batch is idle here: built members join
+                    self._join(f.result())
+                if self.live:
+                    tb = time.perf_counter()
+                    self.batch.launch(self.chunk)
+                    self.n_advance += 1
+                    self.batch.collect()
+                    tc = time.perf_counter()
+                    self.t_advance += tc - tb
+                    self._service_stops(now)
+                    self.t_stops += time.perf_counter() - tc
+                elif building:
+                    ta = time.perf_counter()
+                    building[0].result()  # nothing to step: wait for the first build
+                    self.t_admit += time.perf_counter() - ta
+                elif pending and not waiting:
+                    time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
+                if log and now() - last_log > 10.0:
+                    last_log = now()
+                    log(f"[serve] t={last_log:.0f}s done={len(self.results)} "
+                        f"live={len(self.live)} building={len(building)} waiting={len(waiting)} "
+                        f"pending={len(pending)} resident={self.resident / 1e9:.1f} GB")
+        finally:
+            pool.shutdown(wait=True)
         wall = now()
         ttr = np.array([r.time_to_result for r in self.results]) if self.results else np.zeros(1)
         bstats = self.batch.stats()
@@ -307,8 +316,8 @@ class Server:
             "p99_s": float(np.percentile(ttr, 99)),
             "peak_live": self.peak_live,
             "peak_resident_gb": self.peak_bytes / 1e9,
-            "time_admit_s": self.t_admit,
-            "time_wait_steps_s": self.t_advance,
+            "time_idle_for_builds_s": self.t_admit,
+            "time_stepping_s": self.t_advance,
             "time_stops_s": self.t_stops,
             "advances": self.n_advance,
             **{f"batch_{k}": v for k, v in bstats.items()},
