@@ -149,14 +149,17 @@ static int capture(rbs_batch *b) {
 
 extern "C" {
 
+static int batch_destroy(rbs_batch *b);
+
 int rbs_batch_create(int32_t capacity, rbs_batch **out) {
+  RBS_API_SHARED;
   if (!out || capacity < 1 || capacity > 65536) return set_error("bad batch capacity"), RBS_E_ARG;
   rbs_batch *b = new rbs_batch();
   b->capacity = capacity;
   b->slot.assign(capacity, nullptr);
   b->active.assign(capacity, 0);
   auto fail = [&](int rc) {
-    rbs_batch_destroy(b);
+    batch_destroy(b);
     return rc;
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
@@ -176,6 +179,11 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
 }
 
 int rbs_batch_destroy(rbs_batch *b) {
+  RBS_API_SHARED;
+  return batch_destroy(b);
+}
+
+static int batch_destroy(rbs_batch *b) {
   if (!b) return RBS_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
@@ -191,6 +199,7 @@ int rbs_batch_destroy(rbs_batch *b) {
 
 int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t_stop,
                   int32_t *slot) {
+  RBS_API_SHARED;
   if (!b || !st || !p || !slot) return set_error("null argument"), RBS_E_ARG;
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   if (p->scheme != RBS_SCHEME_LI) return set_error("a batch advances local-inertial scenarios"), RBS_E_ARG;
@@ -213,6 +222,7 @@ int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t
 }
 
 int rbs_batch_remove(rbs_batch *b, int32_t slot) {
+  RBS_API_SHARED;
   if (!b || slot < 0 || slot >= b->capacity || !b->slot[slot])
     return set_error("no member in slot %d", slot), RBS_E_ARG;
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
@@ -233,18 +243,20 @@ int rbs_batch_launch(rbs_batch *b, int32_t steps) {
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   const auto t0 = std::chrono::steady_clock::now();
   bool rebuilt = false;
-  if (b->dirty) {
-    int rc = rebuild(b);
-    if (rc) return rc;
+  if (b->dirty || (!b->graph && !b->order.empty())) {
+    RBS_API_CAPTURE;  // no other thread's runtime calls during the table rebuild + capture
+    if (b->dirty) {
+      int rc = rebuild(b);
+      if (rc) return rc;
+    }
+    if (!b->order.empty() && !b->graph) {
+      int rc = capture(b);
+      if (rc) return rc;
+    }
     rebuilt = true;
   }
   const int n = (int)b->order.size();
   if (n == 0) return RBS_OK;
-  if (!b->graph) {
-    int rc = capture(b);
-    if (rc) return rc;
-    rebuilt = true;
-  }
   if (rebuilt) {
     b->rebuilds += 1;
     b->rebuild_ms +=

@@ -10,6 +10,8 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 
 #include "../../include/rbs.h"
@@ -27,6 +29,15 @@ constexpr double kEpsT = 1e-6;
 constexpr float kSevenThirds = (float)(7.0 / 3.0);
 
 constexpr int kThreads = 256;  // block size of every tiled kernel
+
+// ---- threads ----------------------------------------------------------------
+// Distinct handles may be driven from distinct host threads.  A stream capture
+// (device-driven runs, batches) must not overlap another thread's allocation, copy
+// or launch through the runtime, so every entry point that issues HIP work holds this
+// lock shared and the capture sections hold it exclusively (for a few milliseconds).
+std::shared_mutex &api_mutex();
+#define RBS_API_SHARED std::shared_lock<std::shared_mutex> rbs_api_lock_(::rbs::api_mutex())
+#define RBS_API_CAPTURE std::unique_lock<std::shared_mutex> rbs_api_lock_(::rbs::api_mutex())
 
 // ---- error plumbing ---------------------------------------------------------
 void set_error(const char *fmt, ...);

@@ -230,6 +230,7 @@ extern "C" {
 
 int rbs_set_inflows(rbs_state *s, int32_t count, const int32_t *rows, const int32_t *cols,
                     const int32_t *offsets, const double *knot_t, const double *knot_q) {
+  RBS_API_SHARED;
   if (!s || count < 0) return set_error("bad arguments"), RBS_E_ARG;
   if (count > 0 && (!rows || !cols || !offsets || !knot_t || !knot_q))
     return set_error("null inflow table"), RBS_E_ARG;
@@ -286,6 +287,7 @@ int rbs_set_inflows(rbs_state *s, int32_t count, const int32_t *rows, const int3
 }
 
 int rbs_inflow_apply(rbs_state *s, double t, double dt, double *volume_out) {
+  RBS_API_SHARED;
   if (!s || !volume_out) return set_error("null argument"), RBS_E_ARG;
   *volume_out = 0.0;
   if (s->inflow.count < 1) return RBS_OK;
@@ -299,6 +301,7 @@ int rbs_inflow_apply(rbs_state *s, double t, double dt, double *volume_out) {
 }
 
 int rbs_sed_totals(rbs_state *s, double *totals) {
+  RBS_API_SHARED;
   if (!s || !totals) return set_error("null argument"), RBS_E_ARG;
   if (!(s->armed & RBS_ARM_SED))
     return set_error("bed-change totals need a state armed with RBS_ARM_SED"), RBS_E_STATE;
@@ -321,6 +324,7 @@ int rbs_sed_totals(rbs_state *s, double *totals) {
 }
 
 int rbs_exner_activate(rbs_state *s, double porosity) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (!(s->armed & RBS_ARM_SED))
     return set_error("morphology needs a state armed with RBS_ARM_SED"), RBS_E_STATE;

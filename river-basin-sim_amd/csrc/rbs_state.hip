@@ -11,6 +11,11 @@ namespace rbs {
 
 static thread_local char g_err[512] = "";
 
+std::shared_mutex &api_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
+
 void set_error(const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -224,7 +229,10 @@ int rbs_device_count(int32_t *count) {
   return RBS_OK;
 }
 
+static int state_destroy(rbs_state *s);
+
 int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state **out) {
+  RBS_API_SHARED;
   if (!out) return set_error("null out"), RBS_E_ARG;
   *out = nullptr;
   if (ny < 1 || nx < 1 || !(dx > 0.0) || !std::isfinite(dx)) {
@@ -242,7 +250,7 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
   s->pitch = ((int64_t)nx + 1 + 63) / 64 * 64;
   int rc = RBS_OK;
   auto fail = [&](int code) {
-    rbs_state_destroy(s);
+    state_destroy(s);
     return code;
   };
   if (hipGetDevice(&s->device) != hipSuccess) return fail(hip_fail(hipErrorNoDevice, "hipGetDevice"));
@@ -278,6 +286,11 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
 }
 
 int rbs_state_destroy(rbs_state *s) {
+  RBS_API_SHARED;
+  return state_destroy(s);
+}
+
+static int state_destroy(rbs_state *s) {
   if (!s) return RBS_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   invalidate_graph(s);
@@ -303,6 +316,7 @@ int rbs_state_destroy(rbs_state *s) {
 }
 
 int rbs_arm(rbs_state *s, uint32_t arm) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (arm & ~0x1Fu) return set_error("unknown arm flags 0x%x", arm), RBS_E_ARG;
   return arm_fields(s, arm);
@@ -321,6 +335,7 @@ int rbs_state_bytes(const rbs_state *s, uint64_t *bytes) {
 }
 
 int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
+  RBS_API_SHARED;
   if (!s || !host) return set_error("null argument"), RBS_E_ARG;
   FieldView v;
   int rc = field_view(s, field, &v);
@@ -344,21 +359,28 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
   return RBS_OK;
 }
 
+static int download_field(rbs_state *s, int32_t field, void *host, size_t bytes);
+
 int rbs_download(rbs_state *s, int32_t field, void *host, size_t bytes) {
+  RBS_API_SHARED;
   if (!s || !host) return set_error("null argument"), RBS_E_ARG;
   if (field == RBS_F_ETA) {
     // eta = h + z (local_inertial.py:61-69), formed on the host from the two fields.
     const size_t want = (size_t)s->ny * s->nx * 4;
     if (bytes != want) return set_error("eta expects %zu bytes", want), RBS_E_ARG;
     std::vector<float> z((size_t)s->ny * s->nx);
-    int rc = rbs_download(s, RBS_F_H, host, bytes);
+    int rc = download_field(s, RBS_F_H, host, bytes);
     if (rc) return rc;
-    rc = rbs_download(s, RBS_F_Z, z.data(), bytes);
+    rc = download_field(s, RBS_F_Z, z.data(), bytes);
     if (rc) return rc;
     float *e = (float *)host;
     for (size_t k = 0; k < z.size(); ++k) e[k] = e[k] + z[k];
     return RBS_OK;
   }
+  return download_field(s, field, host, bytes);
+}
+
+static int download_field(rbs_state *s, int32_t field, void *host, size_t bytes) {
   FieldView v;
   int rc = field_view(s, field, &v);
   if (rc) return rc;
@@ -374,6 +396,7 @@ int rbs_download(rbs_state *s, int32_t field, void *host, size_t bytes) {
 }
 
 int rbs_set_boundaries(rbs_state *s, const int32_t edge_types[4]) {
+  RBS_API_SHARED;
   if (!s || !edge_types) return set_error("null argument"), RBS_E_ARG;
   for (int e = 0; e < 4; ++e)
     if (edge_types[e] < RBS_BC_CLOSED || edge_types[e] > RBS_BC_FIXED_STAGE)
@@ -385,6 +408,7 @@ int rbs_set_boundaries(rbs_state *s, const int32_t edge_types[4]) {
 
 int rbs_set_stage_curve(rbs_state *s, int32_t edge, int32_t n, const double *t,
                         const double *level) {
+  RBS_API_SHARED;
   if (!s || edge < 0 || edge > 3 || n < 1 || !t || !level)
     return set_error("bad stage curve arguments"), RBS_E_ARG;
   for (int k = 1; k < n; ++k)
@@ -408,6 +432,7 @@ int rbs_set_stage_curve(rbs_state *s, int32_t edge, int32_t n, const double *t,
 }
 
 int rbs_compute_dt(rbs_state *s, int32_t scheme, double alpha, double dt_max, double *dt_out) {
+  RBS_API_SHARED;
   if (!s || !dt_out) return set_error("null argument"), RBS_E_ARG;
   if (scheme != RBS_SCHEME_LI && scheme != RBS_SCHEME_HLLC)
     return set_error("unknown scheme %d", scheme), RBS_E_ARG;
@@ -447,6 +472,7 @@ int rbs_compute_dt(rbs_state *s, int32_t scheme, double alpha, double dt_max, do
 
 int rbs_li_step(rbs_state *s, double dt, double rain, int32_t limit, double rain_scale,
                 double t) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (!(dt > 0.0) || !std::isfinite(dt)) return set_error("non-positive timestep %g", dt), RBS_E_RANGE;
   int rc = set_scalars_launch(s, t, dt, (float)rain, (float)rain_scale);
@@ -475,6 +501,7 @@ static double stage_at_host(const StageCurve &c, double x) {
 }
 
 int rbs_hllc_step(rbs_state *s, double dt, double rain, double rain_scale, double t) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (!(dt > 0.0) || !std::isfinite(dt)) return set_error("non-positive timestep %g", dt), RBS_E_RANGE;
   if (!(s->armed & RBS_ARM_HLLC)) {
@@ -498,6 +525,7 @@ int rbs_hllc_step(rbs_state *s, double dt, double rain, double rain_scale, doubl
 }
 
 int rbs_ledger_read(rbs_state *s, rbs_ledger *out) {
+  RBS_API_SHARED;
   if (!s || !out) return set_error("null argument"), RBS_E_ARG;
   if (s->speed_scheme == -1) {  // volume of an uploaded state not yet reduced
     int rc = reduce_speed_launch(s, (s->armed & RBS_ARM_HLLC) ? RBS_SCHEME_HLLC : RBS_SCHEME_LI, FIN_PRIME);
@@ -525,6 +553,7 @@ int rbs_ledger_read(rbs_state *s, rbs_ledger *out) {
 }
 
 int rbs_clock_reset(rbs_state *s, double t) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   hipLaunchKernelGGL(clock_reset_kernel, dim3(1), dim3(1), 0, s->stream, s->ctl, t);
   RBS_HIP(hipGetLastError());
@@ -539,6 +568,7 @@ int rbs_sync(rbs_state *s) {
 }
 
 int rbs_set_li_variant(rbs_state *s, int32_t variant) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (variant < 0 || variant >= li_variant_count())
     return set_error("unknown LI variant %d", variant), RBS_E_ARG;
@@ -549,6 +579,7 @@ int rbs_set_li_variant(rbs_state *s, int32_t variant) {
 
 int rbs_time_li_kernel(rbs_state *s, int32_t reps, double dt, double rain, double rain_scale,
                        double *avg_ms) {
+  RBS_API_SHARED;
   if (!s || !avg_ms || reps < 1) return set_error("bad arguments"), RBS_E_ARG;
   if (!(dt > 0.0)) return set_error("non-positive timestep %g", dt), RBS_E_RANGE;
   std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
@@ -700,6 +731,7 @@ int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t m
 
 extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_stop,
                              int64_t max_steps, int64_t *steps_taken) {
+  std::shared_lock<std::shared_mutex> api(api_mutex());
   int64_t steps0 = 0;
   if (max_steps <= 0) max_steps = INT64_MAX / 4;
   int rc = driven_setup(s, p, t_stop, max_steps, &steps0);
@@ -732,7 +764,12 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
     int64_t launches = (want + chunk - 1) / chunk;
     launches = std::max<int64_t>(1, std::min<int64_t>(launches, 64));
     if (!s->graph[0]) {
-      rc = build_graph(s, p->scheme, p->limit, chunk);
+      api.unlock();
+      {
+        RBS_API_CAPTURE;  // no other thread's runtime calls during the capture
+        rc = build_graph(s, p->scheme, p->limit, chunk);
+      }
+      api.lock();
       if (rc) return rc;
     }
     for (int64_t L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(s->graph[0], s->stream));

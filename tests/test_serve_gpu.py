@@ -70,3 +70,28 @@ def test_plain_launches_equal_device_driven_solo():
         assert a.steps == b.steps
         assert np.array_equal(a.h, b.h), f"request {r.rid} ({r.kind}) differs"
     assert kinds == {"basin", "reach"}
+
+
+def test_burst_builds_overlap_stepping_and_results_equal_solo():
+    """Every request arrives at once: states are built on the preparation thread while
+    the batch captures and steps on the main thread; each result equals its solo run."""
+    import dataclasses
+
+    from rbs.serve import InputPool, Server, make_requests, solo
+
+    pool = InputPool(n=512, seed=21)
+    reqs = make_requests(12, rate_per_s=10.0, seed=21, pool=pool, end_time=300.0,
+                         basin_min=96, basin_max=320, reach_scale=0.125)
+    reqs = [dataclasses.replace(r, arrival_s=0.0) for r in reqs]
+    for r in reqs:
+        if r.kind == "reach":
+            r.interval_s = 100.0
+    srv = Server(capacity=16, chunk=16, keep_results=True, prep_threads=2)
+    stats = srv.serve(reqs)
+    assert stats["completed"] == len(reqs) and stats["rejected"] == 0
+    assert stats["batch_rebuilds"] >= 2
+    by_id = {r.rid: r for r in srv.results}
+    for req in reqs:
+        alone = solo(req)
+        assert by_id[req.rid].steps == alone.steps
+        assert np.array_equal(by_id[req.rid].h, alone.h), f"request {req.rid} ({req.kind})"
