@@ -334,8 +334,32 @@ int rbs_state_bytes(const rbs_state *s, uint64_t *bytes) {
   return RBS_OK;
 }
 
+// Host<->device field copies move in blocks of rows of at most kCopyChunk bytes,
+// each under the API lock on its own, so a graph capture on another thread never
+// waits for more than one block.
+constexpr size_t kCopyChunk = size_t(4) << 20;
+
+static int copy_rows(rbs_state *s, const FieldView &v, void *host, bool to_device) {
+  const size_t row_bytes = (size_t)v.cols * v.elem;
+  const size_t dpitch = (size_t)s->pitch * v.elem;
+  const int rows_per = (int)std::max<size_t>(1, kCopyChunk / row_bytes);
+  for (int r0 = 0; r0 < v.rows; r0 += rows_per) {
+    const int nr = std::min(rows_per, v.rows - r0);
+    char *h = (char *)host + (size_t)r0 * row_bytes;
+    char *d = (char *)v.dev + (size_t)r0 * dpitch;
+    RBS_API_SHARED;
+    if (to_device)
+      RBS_HIP(hipMemcpy2DAsync(d, dpitch, h, row_bytes, row_bytes, nr, hipMemcpyHostToDevice,
+                               s->stream));
+    else
+      RBS_HIP(hipMemcpy2DAsync(h, row_bytes, d, dpitch, row_bytes, nr, hipMemcpyDeviceToHost,
+                               s->stream));
+    RBS_HIP(hipStreamSynchronize(s->stream));
+  }
+  return RBS_OK;
+}
+
 int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
-  RBS_API_SHARED;
   if (!s || !host) return set_error("null argument"), RBS_E_ARG;
   FieldView v;
   int rc = field_view(s, field, &v);
@@ -345,9 +369,8 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
     set_error("field %d expects %zu bytes, got %zu", field, want, bytes);
     return RBS_E_ARG;
   }
-  RBS_HIP(hipMemcpy2DAsync(v.dev, (size_t)s->pitch * v.elem, host, (size_t)v.cols * v.elem,
-                           (size_t)v.cols * v.elem, v.rows, hipMemcpyHostToDevice, s->stream));
-  RBS_HIP(hipStreamSynchronize(s->stream));
+  rc = copy_rows(s, v, const_cast<void *>(host), true);
+  if (rc) return rc;
   if (field == RBS_F_H || field == RBS_F_HU || field == RBS_F_HV) s->speed_scheme = -1;
   if (field == RBS_F_RAIN) {
     // run.py:276: the ledger's field-rain inflow uses sum f64(field)
@@ -362,7 +385,6 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
 static int download_field(rbs_state *s, int32_t field, void *host, size_t bytes);
 
 int rbs_download(rbs_state *s, int32_t field, void *host, size_t bytes) {
-  RBS_API_SHARED;
   if (!s || !host) return set_error("null argument"), RBS_E_ARG;
   if (field == RBS_F_ETA) {
     // eta = h + z (local_inertial.py:61-69), formed on the host from the two fields.
@@ -389,10 +411,7 @@ static int download_field(rbs_state *s, int32_t field, void *host, size_t bytes)
     set_error("field %d expects %zu bytes, got %zu", field, want, bytes);
     return RBS_E_ARG;
   }
-  RBS_HIP(hipMemcpy2DAsync(host, (size_t)v.cols * v.elem, v.dev, (size_t)s->pitch * v.elem,
-                           (size_t)v.cols * v.elem, v.rows, hipMemcpyDeviceToHost, s->stream));
-  RBS_HIP(hipStreamSynchronize(s->stream));
-  return RBS_OK;
+  return copy_rows(s, v, host, false);
 }
 
 int rbs_set_boundaries(rbs_state *s, const int32_t edge_types[4]) {
