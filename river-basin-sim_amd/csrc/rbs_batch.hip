@@ -33,6 +33,11 @@ struct rbs_batch {
   int *d_tile_member = nullptr, *d_tile_first = nullptr;
   int tile_cap = 0;
   int total_tiles = 0;
+  BatchDims *d_dims = nullptr;  // the live counts the captured kernels read
+  bool any_sed = false, any_inflow = false;
+  // what the current graph was captured for: launch sizes and optional nodes
+  int graph_tiles = 0;
+  bool graph_sed = false, graph_inflow = false;
   std::vector<int> order;      // compact index -> slot
   std::vector<int> active;     // per slot: 1 running, 0 done
   hipGraphExec_t graph = nullptr;
@@ -53,31 +58,41 @@ static void drop_graph(rbs_batch *b) {
   }
 }
 
+// Rewrite the member tables for the current membership.  The graph is kept: its
+// kernels read the live counts from d_dims.  It is dropped (and captured again at the
+// next launch) only when the membership outgrows what it was captured for.
 static int rebuild(rbs_batch *b) {
-  drop_graph(b);
   b->order.clear();
   for (int k = 0; k < b->capacity; ++k)
     if (b->slot[k] && b->active[k]) b->order.push_back(k);
   const int n = (int)b->order.size();
   b->total_tiles = 0;
   b->plain_tiles = 0;
+  b->n_inflow = 0;
   b->dirty = false;
-  if (n == 0) return RBS_OK;
   const size_t asz = li_args_size();
-  std::vector<char> args(asz * n);
-  std::vector<FinArgs> fin(n);
+  std::vector<char> args(asz * (n > 0 ? n : 1));
+  std::vector<FinArgs> fin(n > 0 ? n : 1);
   // two segments of the flat tile space, each launched with its own kernel form:
   // members without the bedload integral first, then those with it; first[i] is
   // member i's first tile within its segment
-  std::vector<int> first(n), member, member_sed;
+  std::vector<int> first(n > 0 ? n : 1), member, member_sed;
+  std::vector<InflowArgs> inflow;
   for (int i = 0; i < n; ++i) {
     rbs_state *s = b->slot[b->order[i]];
     s->li_variant = b->variant;
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
-    std::vector<int> &seg = (s->armed & RBS_ARM_SED) ? member_sed : member;
+    const bool sed = (s->armed & RBS_ARM_SED) != 0;
+    std::vector<int> &seg = sed ? member_sed : member;
     first[i] = (int)seg.size();
     for (int t = 0; t < nt; ++t) seg.push_back(i);
+    if (sed) b->any_sed = true;
+    if (s->inflow.count > 0) {
+      inflow.emplace_back();
+      inflow_fill_args(s, &inflow.back());
+      b->any_inflow = true;
+    }
     fin[i].ctl = s->ctl;
     fin[i].part = s->part;
     fin[i].sync = s->d_sync;
@@ -86,53 +101,58 @@ static int rebuild(rbs_batch *b) {
     fin[i].dx = s->dx;
     fin[i].nblk = nt;
   }
-  std::vector<InflowArgs> inflow;
-  for (int i = 0; i < n; ++i) {
-    rbs_state *s = b->slot[b->order[i]];
-    if (s->inflow.count > 0) {
-      inflow.emplace_back();
-      inflow_fill_args(s, &inflow.back());
-    }
-  }
   b->n_inflow = (int)inflow.size();
   b->plain_tiles = (int)member.size();
   b->total_tiles = (int)(member.size() + member_sed.size());
   member.insert(member.end(), member_sed.begin(), member_sed.end());
-  if (b->total_tiles > b->tile_cap) {
+  if (b->total_tiles > b->tile_cap) {  // grow by half again: few reallocations
+    RBS_HIP(hipStreamSynchronize(b->stream));
     if (b->d_tile_member) RBS_HIP(hipFree(b->d_tile_member));
     b->d_tile_member = nullptr;
-    RBS_HIP(hipMalloc(&b->d_tile_member, sizeof(int) * b->total_tiles));
-    b->tile_cap = b->total_tiles;
+    const int cap = std::max(b->total_tiles, b->tile_cap + b->tile_cap / 2);
+    RBS_HIP(hipMalloc(&b->d_tile_member, sizeof(int) * cap));
+    b->tile_cap = cap;
+    drop_graph(b);  // it holds the old table
   }
-  RBS_HIP(hipMemcpyAsync(b->d_args, args.data(), asz * n, hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipMemcpyAsync(b->d_fin, fin.data(), sizeof(FinArgs) * n, hipMemcpyHostToDevice, b->stream));
+  if (b->graph && (b->total_tiles > b->graph_tiles || (b->any_sed && !b->graph_sed) ||
+                   (b->any_inflow && !b->graph_inflow)))
+    drop_graph(b);
+  const BatchDims dims{b->plain_tiles, b->total_tiles, n, b->n_inflow};
+  if (n > 0) {
+    RBS_HIP(hipMemcpyAsync(b->d_args, args.data(), asz * n, hipMemcpyHostToDevice, b->stream));
+    RBS_HIP(hipMemcpyAsync(b->d_fin, fin.data(), sizeof(FinArgs) * n, hipMemcpyHostToDevice,
+                           b->stream));
+    RBS_HIP(hipMemcpyAsync(b->d_tile_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                           b->stream));
+    RBS_HIP(hipMemcpyAsync(b->d_tile_member, member.data(), sizeof(int) * member.size(),
+                           hipMemcpyHostToDevice, b->stream));
+  }
   if (b->n_inflow > 0)
     RBS_HIP(hipMemcpyAsync(b->d_inflow, inflow.data(), sizeof(InflowArgs) * b->n_inflow,
                            hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipMemcpyAsync(b->d_tile_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipMemcpyAsync(b->d_tile_member, member.data(), sizeof(int) * member.size(),
-                         hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipStreamSynchronize(b->stream));
+  RBS_HIP(hipMemcpyAsync(b->d_dims, &dims, sizeof(dims), hipMemcpyHostToDevice, b->stream));
+  RBS_HIP(hipStreamSynchronize(b->stream));  // the host vectors die here
   return RBS_OK;
 }
 
+// Capture `chunk` batched steps with launch sizes covering the tile capacity and the
+// member capacity; the kernels read the live counts from d_dims.
 static int capture(rbs_batch *b) {
-  const int n = (int)b->order.size();
+  const int grid_tiles = b->tile_cap;
+  const bool sed = b->any_sed, inflow = b->any_inflow;
   RBS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBS_OK;
   for (int k = 0; k < b->chunk && rc == RBS_OK; ++k) {
     // point sources of the members that have them (inflow.py:167-200), one launch
-    rc = inflow_batch_launch(b->d_inflow, b->n_inflow, b->stream);
+    if (inflow) rc = inflow_batch_launch(b->d_inflow, b->d_dims, b->capacity, b->stream);
     if (rc == RBS_OK)
-      rc = b->plain_tiles > 0
-               ? li_batch_launch(b->variant, false, (const LiArgs *)b->d_args, b->d_tile_member,
-                                 b->d_tile_first, b->plain_tiles, b->stream)
-               : RBS_OK;
-    if (rc == RBS_OK && b->total_tiles > b->plain_tiles)
-      rc = li_batch_launch(b->variant, true, (const LiArgs *)b->d_args,
-                           b->d_tile_member + b->plain_tiles, b->d_tile_first,
-                           b->total_tiles - b->plain_tiles, b->stream);
-    if (rc == RBS_OK) rc = batch_finalize_launch(b->d_fin, n, b->d_status, b->stream);
+      rc = li_batch_launch(b->variant, false, (const LiArgs *)b->d_args, b->d_tile_member,
+                           b->d_tile_first, b->d_dims, grid_tiles, b->stream);
+    if (rc == RBS_OK && sed)
+      rc = li_batch_launch(b->variant, true, (const LiArgs *)b->d_args, b->d_tile_member,
+                           b->d_tile_first, b->d_dims, grid_tiles, b->stream);
+    if (rc == RBS_OK)
+      rc = batch_finalize_launch(b->d_fin, b->d_dims, b->capacity, b->d_status, b->stream);
   }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -144,6 +164,9 @@ static int capture(rbs_batch *b) {
   e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  b->graph_tiles = grid_tiles;
+  b->graph_sed = sed;
+  b->graph_inflow = inflow;
   return RBS_OK;
 }
 
@@ -169,6 +192,7 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
   if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
       hipMalloc(&b->d_fin, sizeof(FinArgs) * capacity) != hipSuccess ||
       hipMalloc(&b->d_inflow, sizeof(InflowArgs) * capacity) != hipSuccess ||
+      hipMalloc(&b->d_dims, sizeof(BatchDims)) != hipSuccess ||
       hipMalloc(&b->d_status, sizeof(int32_t) * capacity) != hipSuccess ||
       hipMalloc(&b->d_tile_first, sizeof(int) * capacity) != hipSuccess)
     return fail(hip_fail(hipErrorOutOfMemory, "hipMalloc(batch tables)"));
@@ -187,7 +211,8 @@ static int batch_destroy(rbs_batch *b) {
   if (!b) return RBS_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  void *p[] = {b->d_args, b->d_fin, b->d_inflow, b->d_status, b->d_tile_member, b->d_tile_first};
+  void *p[] = {b->d_args, b->d_fin, b->d_inflow, b->d_dims, b->d_status, b->d_tile_member,
+               b->d_tile_first};
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->h_status) (void)hipHostFree(b->h_status);
@@ -243,16 +268,16 @@ int rbs_batch_launch(rbs_batch *b, int32_t steps) {
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   const auto t0 = std::chrono::steady_clock::now();
   bool rebuilt = false;
-  if (b->dirty || (!b->graph && !b->order.empty())) {
-    RBS_API_CAPTURE;  // no other thread's runtime calls during the table rebuild + capture
-    if (b->dirty) {
-      int rc = rebuild(b);
-      if (rc) return rc;
-    }
-    if (!b->order.empty() && !b->graph) {
-      int rc = capture(b);
-      if (rc) return rc;
-    }
+  if (b->dirty) {
+    RBS_API_SHARED;
+    int rc = rebuild(b);
+    if (rc) return rc;
+    rebuilt = true;
+  }
+  if (!b->order.empty() && !b->graph) {
+    RBS_API_CAPTURE;  // no other thread's runtime calls during the capture
+    int rc = capture(b);
+    if (rc) return rc;
     rebuilt = true;
   }
   const int n = (int)b->order.size();
@@ -342,7 +367,8 @@ int rbs_batch_set_variant(rbs_batch *b, int32_t variant) {
     return set_error("LI variant %d is not in [0, %d)", variant, li_variant_count()), RBS_E_ARG;
   if (variant != b->variant) {
     b->variant = variant;
-    b->dirty = true;  // tables and graph follow the new tile shape
+    b->dirty = true;  // tables follow the new tile shape ...
+    drop_graph(b);    // ... and the graph launches the new kernels
   }
   return RBS_OK;
 }

@@ -200,9 +200,12 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(DevCtl *ctl, Part
 }
 
 // One block per batch member: that member's device-driven finalize, then its
-// status word (bit 0: still active, bit 1: current LI buffer) for the host.
+// status word (bit 0: still active, bit 1: current LI buffer) for the host.  The grid
+// covers the batch's capacity; blocks past the live member count return at once.
 __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const FinArgs *fin,
+                                                                  const BatchDims *dims,
                                                                   int32_t *status) {
+  if ((int)blockIdx.x >= dims->members) return;
   const FinArgs f = fin[blockIdx.x];
   finalize_body<kThreads>(f.ctl, f.part, f.nblk, f.area, f.n_cells, f.sync, f.dx, FIN_DRIVEN, 1);
   __syncthreads();
@@ -210,8 +213,10 @@ __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const FinArgs 
                                              (f.ctl->active == -1 ? 4 : 0);
 }
 
-int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream) {
-  hipLaunchKernelGGL(batch_finalize_kernel, dim3(n), dim3(kThreads), 0, stream, fin, status);
+int batch_finalize_launch(const FinArgs *fin, const BatchDims *dims, int grid, int32_t *status,
+                          hipStream_t stream) {
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3(grid), dim3(kThreads), 0, stream, fin, dims,
+                     status);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

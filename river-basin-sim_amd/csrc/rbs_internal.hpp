@@ -95,6 +95,16 @@ struct Partials {
 };
 
 // What the batched finalize needs of one member.
+// The size of a batch's current membership, read by the batched kernels from device
+// memory: a captured graph launches grids sized for the batch's capacity, the blocks
+// past these counts return at once, and a membership change only rewrites tables.
+struct BatchDims {
+  int32_t plain_tiles;     // tile segment 0: members without the bedload integral
+  int32_t total_tiles;     // segment 1 (bedload members) is [plain_tiles, total_tiles)
+  int32_t members;         // finalize blocks
+  int32_t inflow_members;  // members with point sources
+};
+
 // What one member's point-source step needs (device-driven or host-driven).
 struct InflowArgs {
   float *h0, *h1;  // the LI ping-pong pair (the same buffer twice for HLLC)
@@ -206,11 +216,13 @@ int ensure_partials(rbs_state *s, int nblk);
 
 int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
                  int64_t *steps0);
-int batch_finalize_launch(const FinArgs *fin, int n, int32_t *status, hipStream_t stream);
+int batch_finalize_launch(const FinArgs *fin, const BatchDims *dims, int grid, int32_t *status,
+                          hipStream_t stream);
 void inflow_fill_args(const rbs_state *s, InflowArgs *out);
-int inflow_batch_launch(const InflowArgs *list, int n, hipStream_t stream);
+int inflow_batch_launch(const InflowArgs *list, const BatchDims *dims, int grid,
+                        hipStream_t stream);
 int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
-                    const int *tile_first, int total, hipStream_t stream);
+                    const int *tile_first, const BatchDims *dims, int grid, hipStream_t stream);
 size_t li_args_size();
 void li_fill_args(const rbs_state *s, int limit, LiArgs *out);
 

@@ -532,33 +532,46 @@ void li_strip_kernel(LiArgs a) {
   li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
-// A batch of states in one launch: the flat tile space of all members, mapped back
-// to (member, tile) by a table; each member keeps its own clock, dt and ledger.
+// A batch of states in one launch per tile segment (seg 0: members without the
+// bedload integral, seg 1: with it): the segment's flat tile space, mapped back to
+// (member, tile) by a table; each member keeps its own clock, dt and ledger.  The
+// grid is sized for the batch's capacity; blocks past the segment's live tile count
+// return at once.
+__device__ __forceinline__ bool batch_tile(const int *__restrict__ tile_member,
+                                           const int *__restrict__ tile_first,
+                                           const BatchDims *dims, int seg, int &m, int &tile) {
+  const int base = seg ? dims->plain_tiles : 0;
+  const int total = seg ? dims->total_tiles - dims->plain_tiles : dims->plain_tiles;
+  if ((int)blockIdx.x >= total) return false;
+  const int t = xcd_tile(blockIdx.x, total);
+  // the member index is block-uniform: its arguments are read with scalar loads
+  m = __builtin_amdgcn_readfirstlane(tile_member[base + t]);
+  tile = t - tile_first[m];
+  return true;
+}
+
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__restrict__ members,
                                                             const int *__restrict__ tile_member,
                                                             const int *__restrict__ tile_first,
-                                                            int total) {
-  const int t = xcd_tile(blockIdx.x, total);
-  // the member index is block-uniform: its arguments are read with scalar loads
-  const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
-  li_tile<BX, BY>(members[m], t - tile_first[m]);
+                                                            const BatchDims *dims, int seg) {
+  int m, tile;
+  if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) li_tile<BX, BY>(members[m], tile);
 }
 template <int H, bool SED, int W>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 4 : W)))
 void li_strip_batch_kernel(
     const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
-    const int *__restrict__ tile_first, int total) {
-  const int t = xcd_tile(blockIdx.x, total);
-  const int m = __builtin_amdgcn_readfirstlane(tile_member[t]);
-  li_strip<H, SED>(members[m], t - tile_first[m]);
+    const int *__restrict__ tile_first, const BatchDims *dims, int seg) {
+  int m, tile;
+  if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) li_strip<H, SED>(members[m], tile);
 }
 
 // ---- variants (the autotuner picks one per grid shape) -------------------------------
 // bx x by is the block's tile of cells.  `kern_sed` / `batch_sed` are the forms used
 // when a state carries the bedload integral (the LDS family handles it at run time).
 using StepFn = void (*)(LiArgs);
-using BatchFn = void (*)(const LiArgs *, const int *, const int *, int);
+using BatchFn = void (*)(const LiArgs *, const int *, const int *, const BatchDims *, int);
 struct LiVariant {
   int bx, by;
   int family;  // 0 LDS tile, 1 register strip
@@ -596,10 +609,10 @@ void li_variant_shape(int v, int *bx, int *by, int *family) {
 size_t li_args_size() { return sizeof(LiArgs); }
 
 int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
-                    const int *tile_first, int total, hipStream_t stream) {
+                    const int *tile_first, const BatchDims *dims, int grid, hipStream_t stream) {
   const LiVariant &v = variant_at(variant);
-  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(total), dim3(kThreads), 0, stream,
-                     members, tile_member, tile_first, total);
+  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(grid), dim3(kThreads), 0, stream,
+                     members, tile_member, tile_first, dims, sed ? 1 : 0);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

@@ -68,8 +68,11 @@ __global__ __launch_bounds__(kThreads) void inflow_kernel(InflowArgs a, int driv
   inflow_body(a, driven, t_host, dt_host, out);
 }
 
-// A batch's point sources: one block per member that has any (device-driven).
-__global__ __launch_bounds__(kThreads) void inflow_batch_kernel(const InflowArgs *list) {
+// A batch's point sources: one block per member that has any (device-driven); the
+// grid covers the batch's capacity and blocks past the live count return at once.
+__global__ __launch_bounds__(kThreads) void inflow_batch_kernel(const InflowArgs *list,
+                                                                const BatchDims *dims) {
+  if ((int)blockIdx.x >= dims->inflow_members) return;
   inflow_body(list[blockIdx.x], 1, 0.0, 0.0, nullptr);
 }
 
@@ -93,9 +96,10 @@ void inflow_fill_args(const rbs_state *s, InflowArgs *out) {
   *out = a;
 }
 
-int inflow_batch_launch(const InflowArgs *list, int n, hipStream_t stream) {
-  if (n < 1) return RBS_OK;
-  hipLaunchKernelGGL(inflow_batch_kernel, dim3(n), dim3(kThreads), 0, stream, list);
+int inflow_batch_launch(const InflowArgs *list, const BatchDims *dims, int grid,
+                        hipStream_t stream) {
+  if (grid < 1) return RBS_OK;
+  hipLaunchKernelGGL(inflow_batch_kernel, dim3(grid), dim3(kThreads), 0, stream, list, dims);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }
