@@ -240,10 +240,11 @@ int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *fami
 /* The LI kernel variant a batch launches for all its members (the default unless
  * set); takes effect at the next advance. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant);
-/* Host-side cost of membership changes so far: table rebuilds and graph captures
- * (count and wall milliseconds) and graph replays launched. */
+/* Host-side cost of membership changes so far (table rebuilds and graph captures:
+ * count and wall milliseconds), graph replays launched, and the device time of the
+ * batched steps (HIP events around each advance). */
 int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
-                    int64_t *graph_launches);
+                    int64_t *graph_launches, double *gpu_ms);
 
 #ifdef __cplusplus
 }

@@ -45,8 +45,9 @@ struct rbs_batch {
   // host-side cost of membership changes
   int64_t rebuilds = 0, graph_launches = 0;
   double rebuild_ms = 0.0;
-  // an advance in flight (launched, not yet collected)
-  hipEvent_t done_ev = nullptr;
+  // an advance in flight (launched, not yet collected); the event pair times it
+  hipEvent_t start_ev = nullptr, done_ev = nullptr;
+  double gpu_ms = 0.0;
   bool in_flight = false;
   int in_flight_n = 0;
 };
@@ -187,7 +188,7 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipStreamCreate"));
-  if (hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming) != hipSuccess)
+  if (hipEventCreate(&b->start_ev) != hipSuccess || hipEventCreate(&b->done_ev) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipEventCreate"));
   if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
       hipMalloc(&b->d_fin, sizeof(FinArgs) * capacity) != hipSuccess ||
@@ -217,6 +218,7 @@ static int batch_destroy(rbs_batch *b) {
     if (q) (void)hipFree(q);
   if (b->h_status) (void)hipHostFree(b->h_status);
   if (b->done_ev) (void)hipEventDestroy(b->done_ev);
+  if (b->start_ev) (void)hipEventDestroy(b->start_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RBS_OK;
@@ -288,6 +290,7 @@ int rbs_batch_launch(rbs_batch *b, int32_t steps) {
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   const int launches = (steps + b->chunk - 1) / b->chunk;
+  RBS_HIP(hipEventRecord(b->start_ev, b->stream));
   for (int L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(b->graph, b->stream));
   b->graph_launches += launches;
   RBS_HIP(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
@@ -319,6 +322,8 @@ int rbs_batch_collect(rbs_batch *b, int32_t *n_active) {
   if (b->in_flight) {
     RBS_HIP(hipEventSynchronize(b->done_ev));
     b->in_flight = false;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->start_ev, b->done_ev) == hipSuccess) b->gpu_ms += ms;
     bool changed = false;
     for (int i = 0; i < b->in_flight_n; ++i) {
       const int k = b->order[i];
@@ -352,11 +357,13 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active) {
 }
 
 int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
-                    int64_t *graph_launches) {
-  if (!b || !rebuilds || !rebuild_ms || !graph_launches) return set_error("null argument"), RBS_E_ARG;
+                    int64_t *graph_launches, double *gpu_ms) {
+  if (!b || !rebuilds || !rebuild_ms || !graph_launches || !gpu_ms)
+    return set_error("null argument"), RBS_E_ARG;
   *rebuilds = b->rebuilds;
   *rebuild_ms = b->rebuild_ms;
   *graph_launches = b->graph_launches;
+  *gpu_ms = b->gpu_ms;
   return RBS_OK;
 }
 

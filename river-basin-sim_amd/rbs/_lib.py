@@ -109,7 +109,7 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_batch_collect": (ctypes.c_int, [_vp, _pi32]),
     "rbs_sed_totals": (ctypes.c_int, [_vp, _pdbl]),
     "rbs_batch_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), _pdbl,
-                                       ctypes.POINTER(ctypes.c_int64)]),
+                                       ctypes.POINTER(ctypes.c_int64), _pdbl]),
     "rbs_time_li_kernel": (ctypes.c_int, [_vp, _i32, _dbl, _dbl, _dbl, _pdbl]),
 }
 

@@ -80,11 +80,11 @@ class Batch:
     def stats(self) -> dict:
         """Membership-change cost so far (table rebuilds + graph captures) and replays."""
         r, g = ctypes.c_int64(), ctypes.c_int64()
-        ms = ctypes.c_double()
+        ms, gpu = ctypes.c_double(), ctypes.c_double()
         _lib.call("rbs_batch_stats", self._handle, ctypes.byref(r), ctypes.byref(ms),
-                  ctypes.byref(g))
+                  ctypes.byref(g), ctypes.byref(gpu))
         return {"rebuilds": int(r.value), "rebuild_s": ms.value / 1e3,
-                "graph_launches": int(g.value)}
+                "graph_launches": int(g.value), "gpu_s": gpu.value / 1e3}
 
     def run_to_completion(self, chunk: int = 64) -> None:
         while self.advance(chunk) > 0:
