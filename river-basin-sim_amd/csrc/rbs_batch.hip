@@ -34,6 +34,8 @@ struct rbs_batch {
   int tile_cap = 0;
   int total_tiles = 0;
   BatchDims *d_dims = nullptr;  // the live counts the captured kernels read
+  char *h_stage = nullptr;      // pinned staging of the tables (one rebuild's worth)
+  size_t stage_cap = 0;
   bool any_sed = false, any_inflow = false;
   // what the current graph was captured for: launch sizes and optional nodes
   int graph_tiles = 0;
@@ -119,20 +121,37 @@ static int rebuild(rbs_batch *b) {
                    (b->any_inflow && !b->graph_inflow)))
     drop_graph(b);
   const BatchDims dims{b->plain_tiles, b->total_tiles, n, b->n_inflow};
-  if (n > 0) {
-    RBS_HIP(hipMemcpyAsync(b->d_args, args.data(), asz * n, hipMemcpyHostToDevice, b->stream));
-    RBS_HIP(hipMemcpyAsync(b->d_fin, fin.data(), sizeof(FinArgs) * n, hipMemcpyHostToDevice,
-                           b->stream));
-    RBS_HIP(hipMemcpyAsync(b->d_tile_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice,
-                           b->stream));
-    RBS_HIP(hipMemcpyAsync(b->d_tile_member, member.data(), sizeof(int) * member.size(),
-                           hipMemcpyHostToDevice, b->stream));
+  // stage every table in pinned memory, then copy them with DMA in stream order
+  struct Piece {
+    void *dev;
+    const void *src;
+    size_t bytes;
+  };
+  const Piece pieces[] = {
+      {b->d_args, args.data(), n > 0 ? asz * n : 0},
+      {b->d_fin, fin.data(), sizeof(FinArgs) * n},
+      {b->d_tile_first, first.data(), sizeof(int) * n},
+      {b->d_tile_member, member.data(), sizeof(int) * member.size()},
+      {b->d_inflow, inflow.data(), sizeof(InflowArgs) * inflow.size()},
+      {b->d_dims, &dims, sizeof(dims)},
+  };
+  size_t need = 0;
+  for (const Piece &q : pieces) need += (q.bytes + 255) & ~size_t(255);
+  if (need > b->stage_cap) {
+    if (b->h_stage) RBS_HIP(hipHostFree(b->h_stage));
+    b->h_stage = nullptr;
+    const size_t cap = std::max(need, b->stage_cap + b->stage_cap / 2);
+    RBS_HIP(hipHostMalloc((void **)&b->h_stage, cap));
+    b->stage_cap = cap;
   }
-  if (b->n_inflow > 0)
-    RBS_HIP(hipMemcpyAsync(b->d_inflow, inflow.data(), sizeof(InflowArgs) * b->n_inflow,
-                           hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipMemcpyAsync(b->d_dims, &dims, sizeof(dims), hipMemcpyHostToDevice, b->stream));
-  RBS_HIP(hipStreamSynchronize(b->stream));  // the host vectors die here
+  size_t off = 0;
+  for (const Piece &q : pieces) {
+    if (q.bytes == 0) continue;
+    std::memcpy(b->h_stage + off, q.src, q.bytes);
+    RBS_HIP(hipMemcpyAsync(q.dev, b->h_stage + off, q.bytes, hipMemcpyHostToDevice, b->stream));
+    off += (q.bytes + 255) & ~size_t(255);
+  }
+  RBS_HIP(hipStreamSynchronize(b->stream));  // the staging is reused by the next rebuild
   return RBS_OK;
 }
 
@@ -217,6 +236,7 @@ static int batch_destroy(rbs_batch *b) {
   for (void *q : p)
     if (q) (void)hipFree(q);
   if (b->h_status) (void)hipHostFree(b->h_status);
+  if (b->h_stage) (void)hipHostFree(b->h_stage);
   if (b->done_ev) (void)hipEventDestroy(b->done_ev);
   if (b->start_ev) (void)hipEventDestroy(b->start_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);

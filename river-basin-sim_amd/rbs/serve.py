@@ -227,19 +227,27 @@ class Server:
         self._n_prepared -= 1
         self.peak_live = max(self.peak_live, len(self.live))
 
-    def _service_stops(self, now_fn) -> None:
-        """Members that reached a stop: run their slow processes, rejoin or finish."""
+    def _detach_stopped(self) -> list:
+        """Members that reached a stop leave the batch (the tables follow at the next
+        launch); their stop work runs while the batch steps the others."""
+        out = []
         for slot in [s for s in list(self.live) if not self.batch.active(s)]:
-            lv = self.live[slot]
+            lv = self.live.pop(slot)
             self.batch.remove(slot)
+            out.append(lv)
+        return out
+
+    def _handle_stops(self, stopped: list, now_fn) -> list:
+        """Slow processes at the stop (on the member's own stream), or the result;
+        returns the members that continue."""
+        cont = []
+        for lv in stopped:
             seg = lv.segments[lv.seg_idx]
             for proc, elapsed in seg.due:
                 proc.advance(seg.t_stop, elapsed)
             lv.seg_idx += 1
             if lv.seg_idx < len(lv.segments):
-                lv.slot = self.batch.add(lv.run, lv.segments[lv.seg_idx].t_stop)
-                del self.live[slot]
-                self.live[lv.slot] = lv
+                cont.append(lv)
                 continue
             led = lv.st.ledger()
             h = lv.st.h.numpy() if self.keep else None
@@ -248,7 +256,11 @@ class Server:
                                        lv.admitted_s, done, int(led.steps), h))
             lv.st.close()
             self.resident -= lv.bytes
-            del self.live[slot]
+        return cont
+
+    def _rejoin(self, lv: "_Live") -> None:
+        lv.slot = self.batch.add(lv.run, lv.segments[lv.seg_idx].t_stop)
+        self.live[lv.slot] = lv
 
     def serve(self, requests: list[Request], *, log=None) -> dict:
         """Serve requests whose ``arrival_s`` is wall-clock seconds after start."""
@@ -266,7 +278,8 @@ class Server:
         pool = ThreadPoolExecutor(max_workers=self.prep_threads)
         building: list = []  # futures of _Live, in admission order
         try:
-            while pending or waiting or self.live or building:
+            stopped: list = []  # members detached at a stop, their stop work pending
+            while pending or waiting or self.live or building or stopped:
                 t = now()
                 while pending and pending[0][0] <= t:
                     waiting.append(heapq.heappop(pending)[2])
@@ -281,21 +294,29 @@ class Server:
                 building = [f for f in building if not f.done()]
                 for f in ready:  # the batch is idle here: built members join
                     self._join(f.result())
+                launched = False
                 if self.live:
                     tb = time.perf_counter()
                     self.batch.launch(self.chunk)
                     self.n_advance += 1
+                    launched = True
+                # while the device steps: the stop work of members detached last round
+                tc = time.perf_counter()
+                cont = self._handle_stops(stopped, now) if stopped else []
+                self.t_stops += time.perf_counter() - tc
+                stopped = []
+                if launched:
                     self.batch.collect()
-                    tc = time.perf_counter()
-                    self.t_advance += tc - tb
-                    self._service_stops(now)
-                    self.t_stops += time.perf_counter() - tc
-                elif building:
+                    self.t_advance += time.perf_counter() - tb
+                    stopped = self._detach_stopped()
+                elif building and not cont:
                     ta = time.perf_counter()
                     building[0].result()  # nothing to step: wait for the first build
                     self.t_admit += time.perf_counter() - ta
-                elif pending and not waiting:
+                elif pending and not waiting and not cont:
                     time.sleep(max(0.0, min(0.05, pending[0][0] - now())))
+                for lv in cont:  # the batch is idle here: continuing members rejoin
+                    self._rejoin(lv)
                 if log and now() - last_log > 10.0:
                     last_log = now()
                     log(f"[serve] t={last_log:.0f}s done={len(self.results)} "
