@@ -116,6 +116,7 @@ def main() -> int:
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--rate", type=float, default=4.0, help="served arrivals per second")
+    ap.add_argument("--chunk", type=int, default=64, help="serve: batched steps per advance")
     ap.add_argument("--capacity-sample", type=int, default=120,
                     help="serve: requests replayed as a burst and as plain launches")
     ap.add_argument("--sim-hours", type=float, default=1.0, help="simulated time per scenario")
@@ -406,10 +407,11 @@ def bench_serve(args) -> int:
     reqs = make_requests(args.requests, rate_per_s=args.rate, seed=20261206, pool=pool,
                          end_time=3600.0 * args.sim_hours)
     log(f"[serve] {len(reqs)} requests generated in {time.time() - t0:.0f} s")
-    srv = Server(capacity=256, chunk=32)
+    srv = Server(capacity=256, chunk=args.chunk)
     stats = srv.serve(reqs, log=log)
     log(f"[serve] {stats}")
-    cap = serve_capacity(reqs[: args.capacity_sample]) if args.capacity_sample > 0 else None
+    cap = (serve_capacity(reqs[: args.capacity_sample], args.chunk) if args.capacity_sample > 0
+           else None)
     kinds = {}
     for r in srv.results:
         kinds.setdefault(r.kind, []).append(r.time_to_result)
@@ -444,7 +446,7 @@ def bench_serve(args) -> int:
     return 0
 
 
-def serve_capacity(reqs) -> dict:
+def serve_capacity(reqs, chunk: int) -> dict:
     """Capacity of the server against plain per-request launches on the same requests:
     the server takes them as one burst (all arrive at once); plain runs them one after
     another through the reference's tick loop.  Every served result must equal its
@@ -454,7 +456,7 @@ def serve_capacity(reqs) -> dict:
     from rbs.serve import Server, plain
 
     burst = [dataclasses.replace(r, arrival_s=0.0) for r in reqs]
-    srv = Server(capacity=256, chunk=32, keep_results=True)
+    srv = Server(capacity=256, chunk=chunk, keep_results=True)
     st = srv.serve(burst, log=log)
     served = {r.rid: r for r in srv.results}
     t_plain, same = 0.0, True
