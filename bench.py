@@ -523,6 +523,12 @@ def bench_hllc(args) -> int:
     value = n * n * taken / el / 1e9
     led = st.ledger()
     log(f"[hllc] {taken} steps in {el:.2f} s -> {value:.2f} Gcell/s, t={led.t:.1f}")
+    dt_k = float(led.dt) if led.dt > 0 else 0.5
+    avg = ctypes.c_double()
+    _lib.call("rbs_time_hllc_step", st._handle, 10, dt_k, ctypes.byref(avg))
+    step_us = avg.value * 1e3
+    log(f"[hllc] stage kernels {step_us:.0f} us per step (HIP events)")
+    sweep = hllc_sweep(st, dt_k, n * n) if args.sweep else None
     st.close()
     lake = lake_at_rest(bed, dx)
     bytes_cell = 32
@@ -546,9 +552,35 @@ def bench_hllc(args) -> int:
                      "frac": round(bytes_cell * value / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_cell": bytes_cell},
         "lake_at_rest": lake,
+        "stage_kernels_us_per_step": round(step_us, 1),
     }
+    if sweep:
+        out["variant_sweep"] = sweep
     print(json.dumps(out), flush=True)
     return 0
+
+
+def hllc_sweep(st, dt: float, cells: int) -> list:
+    """Mean duration of both stage kernels for every HLLC tile variant (same state)."""
+    from rbs import _lib
+
+    count = ctypes.c_int32()
+    _lib.call("rbs_hllc_variant_count", ctypes.byref(count))
+    rows = []
+    for v in range(count.value):
+        bx, by, dflt = (ctypes.c_int32() for _ in range(3))
+        _lib.call("rbs_hllc_variant_info", v, ctypes.byref(bx), ctypes.byref(by),
+                  ctypes.byref(dflt))
+        _lib.call("rbs_set_hllc_variant", st._handle, v)
+        avg = ctypes.c_double()
+        _lib.call("rbs_time_hllc_step", st._handle, 2, dt, ctypes.byref(avg))
+        _lib.call("rbs_time_hllc_step", st._handle, 10, dt, ctypes.byref(avg))
+        us = avg.value * 1e3
+        rows.append({"variant": v, "tile": f"{bx.value}x{by.value}", "default": bool(dflt.value),
+                     "step_us": round(us, 1), "gcell_s": round(cells / us / 1e3, 2)})
+        log(f"[hllc] variant {v} {bx.value}x{by.value}: {us:.0f} us per step, "
+            f"{cells / us / 1e3:.1f} Gcell/s")
+    return rows
 
 
 def lake_at_rest(bed, dx: float, side: int = 1024, steps: int = 1000) -> dict:

@@ -237,6 +237,14 @@ int rbs_li_variant_count(int32_t *count);
  * with two columns per lane); *is_default = 1 for the library default. */
 int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *family,
                         int32_t *is_default);
+/* HLLC stage-kernel tile variants (LDS per block against halo overhead), as for LI. */
+int rbs_set_hllc_variant(rbs_state *st, int32_t variant);
+int rbs_hllc_variant_count(int32_t *count);
+int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default);
+/* Run `reps` host-driven HLLC steps of `dt` without sources, timing each pair of stage
+ * kernels with HIP events on the state's stream; *avg_ms receives the mean.  Advances
+ * the state. */
+int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
 /* The LI kernel variant a batch launches for all its members (the default unless
  * set); takes effect at the next advance. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant);

@@ -459,10 +459,37 @@ __global__ __launch_bounds__(kThreads) void hllc_stage_kernel(HlArgs a) {
   }
 }
 
-constexpr int kHBX = 64, kHBY = 16;
+// ---- tile variants: LDS per block (and so occupancy) against halo overhead ---------
+struct HlVariant {
+  int bx, by;
+  void (*stage1)(HlArgs);
+  void (*stage2)(HlArgs);
+};
+#define RBS_HL_TILE(X, Y) {X, Y, hllc_stage_kernel<X, Y, 1>, hllc_stage_kernel<X, Y, 2>}
+static const HlVariant kHlVariants[] = {
+    RBS_HL_TILE(64, 16), RBS_HL_TILE(32, 16), RBS_HL_TILE(64, 8),
+    RBS_HL_TILE(32, 8),  RBS_HL_TILE(16, 16),
+};
+constexpr int kNumHlVariants = sizeof(kHlVariants) / sizeof(kHlVariants[0]);
+constexpr int kDefaultHlVariant = 0;
+
+int hllc_variant_count() { return kNumHlVariants; }
+int hllc_default_variant() { return kDefaultHlVariant; }
+void hllc_variant_shape(int v, int *bx, int *by) {
+  if (v < 0 || v >= kNumHlVariants) v = kDefaultHlVariant;
+  *bx = kHlVariants[v].bx;
+  *by = kHlVariants[v].by;
+}
+
+static const HlVariant &hl_variant(const rbs_state *s) {
+  int v = s->hllc_variant;
+  if (v < 0 || v >= kNumHlVariants) v = kDefaultHlVariant;
+  return kHlVariants[v];
+}
 
 int hllc_num_blocks(const rbs_state *s) {
-  return ((s->nx + kHBX - 1) / kHBX) * ((s->ny + kHBY - 1) / kHBY);
+  const HlVariant &v = hl_variant(s);
+  return ((s->nx + v.bx - 1) / v.bx) * ((s->ny + v.by - 1) / v.by);
 }
 
 int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
@@ -482,7 +509,8 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   a.pitch = s->pitch;
   a.ny = s->ny;
   a.nx = s->nx;
-  a.tiles_x = (s->nx + kHBX - 1) / kHBX;
+  const HlVariant &v = hl_variant(s);
+  a.tiles_x = (s->nx + v.bx - 1) / v.bx;
   a.n_tiles = hllc_num_blocks(s);
   for (int e = 0; e < 4; ++e) a.bc[e] = s->bc[e];
   int f = 0;
@@ -507,8 +535,7 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   a.hn = h;
   a.hun = s->hu;
   a.hvn = s->hv;
-  hipLaunchKernelGGL((hllc_stage_kernel<kHBX, kHBY, 1>), dim3(a.n_tiles), dim3(kThreads), 0,
-                     s->stream, a);
+  hipLaunchKernelGGL(v.stage1, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   // stage 2: U1 (halo) + U^n (own cell) -> U^{n+1} in place
   a.h_in = s->h1;
@@ -517,8 +544,7 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   a.h_out = h;
   a.hu_out = s->hu;
   a.hv_out = s->hv;
-  hipLaunchKernelGGL((hllc_stage_kernel<kHBX, kHBY, 2>), dim3(a.n_tiles), dim3(kThreads), 0,
-                     s->stream, a);
+  hipLaunchKernelGGL(v.stage2, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

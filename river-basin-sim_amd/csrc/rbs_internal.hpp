@@ -156,6 +156,7 @@ struct rbs_state {
   hipStream_t stream = nullptr;
   int32_t bc[4] = {0, 0, 0, 0};
   int32_t li_variant = 0;
+  int32_t hllc_variant = 0;
   // LI ping-pong: index `cur` holds the current state
   int cur = 0;
   float *h[2] = {nullptr, nullptr};
@@ -209,6 +210,9 @@ int inflow_launch(rbs_state *s, bool device_scalars, double t, double dt);
 int exner_launch(rbs_state *s, double porosity);
 int li_num_blocks(const rbs_state *s);
 int hllc_num_blocks(const rbs_state *s);
+int hllc_variant_count();
+int hllc_default_variant();
+void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
 void li_variant_shape(int v, int *bx, int *by, int *family);

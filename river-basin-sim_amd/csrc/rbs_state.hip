@@ -276,7 +276,11 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
   }
   s->li_variant = li_default_variant();
   nblk = std::max(nblk, reduce_partials_needed(s));
-  nblk = std::max(nblk, hllc_num_blocks(s));
+  for (int v = 0; v < hllc_variant_count(); ++v) {
+    s->hllc_variant = v;
+    nblk = std::max(nblk, hllc_num_blocks(s));
+  }
+  s->hllc_variant = hllc_default_variant();
   TRY(ensure_partials(s, nblk));
   if (arm) TRY(arm_fields(s, arm));
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "sync"));
@@ -627,9 +631,72 @@ int rbs_time_li_kernel(rbs_state *s, int32_t reps, double dt, double rain, doubl
   return rc;
 }
 
+int rbs_time_hllc_step(rbs_state *s, int32_t reps, double dt, double *avg_ms) {
+  RBS_API_SHARED;
+  if (!s || !avg_ms || reps < 1) return set_error("bad arguments"), RBS_E_ARG;
+  if (!(dt > 0.0)) return set_error("non-positive timestep %g", dt), RBS_E_RANGE;
+  if (!(s->armed & RBS_ARM_HLLC)) return set_error("state is not armed for HLLC"), RBS_E_STATE;
+  // stage levels as they stand (the ghost values only; timing does not need the curve)
+  float stage[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int e = 0; e < 4; ++e)
+    if (s->bc[e] == RBS_BC_FIXED_STAGE && s->stage[e].n > 0)
+      stage[e] = (float)stage_at_host(s->stage[e], 0.0);
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  int rc = RBS_OK;
+  for (auto &e : ev) RBS_HIP(hipEventCreate(&e));
+  for (int k = 0; k < reps && rc == RBS_OK; ++k) {
+    rc = set_scalars_launch(s, 0.0, dt, 0.0f, 0.0f, stage);
+    if (rc) break;
+    if (hipEventRecord(ev[2 * k], s->stream) != hipSuccess) { rc = hip_fail(hipErrorUnknown, "hipEventRecord"); break; }
+    rc = hllc_launch_step(s, false, dt, 0.0, 0.0, 0.0);  // both SSP-RK2 stage kernels
+    if (rc) break;
+    if (hipEventRecord(ev[2 * k + 1], s->stream) != hipSuccess) { rc = hip_fail(hipErrorUnknown, "hipEventRecord"); break; }
+    rc = finalize_launch(s, FIN_EXPLICIT, RBS_SCHEME_HLLC);
+  }
+  double total = 0.0;
+  if (rc == RBS_OK && hipStreamSynchronize(s->stream) == hipSuccess) {
+    for (int k = 0; k < reps; ++k) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) { rc = hip_fail(hipErrorUnknown, "hipEventElapsedTime"); break; }
+      total += ms;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->speed_scheme = RBS_SCHEME_HLLC;
+  if (rc == RBS_OK) *avg_ms = total / reps;
+  return rc;
+}
+
 int rbs_li_variant_count(int32_t *count) {
   if (!count) return set_error("null count"), RBS_E_ARG;
   *count = li_variant_count();
+  return RBS_OK;
+}
+
+int rbs_set_hllc_variant(rbs_state *s, int32_t variant) {
+  if (!s) return set_error("null state"), RBS_E_ARG;
+  if (variant < 0 || variant >= hllc_variant_count())
+    return set_error("HLLC variant %d is not in [0, %d)", variant, hllc_variant_count()), RBS_E_ARG;
+  s->hllc_variant = variant;
+  invalidate_graph(s);  // a captured device-driven run launches the old tiles
+  return RBS_OK;
+}
+
+int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default) {
+  if (!bx || !by || !is_default) return set_error("null argument"), RBS_E_ARG;
+  if (variant < 0 || variant >= hllc_variant_count())
+    return set_error("HLLC variant %d is not in [0, %d)", variant, hllc_variant_count()), RBS_E_ARG;
+  int x, y;
+  hllc_variant_shape(variant, &x, &y);
+  *bx = x;
+  *by = y;
+  *is_default = variant == hllc_default_variant() ? 1 : 0;
+  return RBS_OK;
+}
+
+int rbs_hllc_variant_count(int32_t *count) {
+  if (!count) return set_error("null count"), RBS_E_ARG;
+  *count = hllc_variant_count();
   return RBS_OK;
 }
 

@@ -104,6 +104,10 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_li_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_li_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32, _pi32]),
     "rbs_batch_set_variant": (ctypes.c_int, [_vp, _i32]),
+    "rbs_set_hllc_variant": (ctypes.c_int, [_vp, _i32]),
+    "rbs_hllc_variant_count": (ctypes.c_int, [_pi32]),
+    "rbs_hllc_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),
+    "rbs_time_hllc_step": (ctypes.c_int, [_vp, _i32, _dbl, _pdbl]),
     "rbs_batch_launch": (ctypes.c_int, [_vp, _i32]),
     "rbs_batch_poll": (ctypes.c_int, [_vp, _pi32]),
     "rbs_batch_collect": (ctypes.c_int, [_vp, _pi32]),

@@ -160,3 +160,37 @@ def test_device_driven_hllc_matches_host_loop(rbs):
     assert np.array_equal(host.h.numpy(), dev.h.numpy())
     assert np.array_equal(host.hu.numpy(), dev.hu.numpy())
     assert host.loss_volume() == pytest.approx(dev.loss_volume(), rel=1e-12)
+
+
+def test_tile_variants_are_bitwise(rbs):
+    """Every HLLC tile variant writes the same bits (open, stage and wall edges, rain
+    field with Kahan debt, infiltration)."""
+    import ctypes
+
+    rng = np.random.default_rng(8)
+    ny, nx = 83, 141  # ragged against every tile shape
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.01 * (nx - xx) + 0.2 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    d0 = np.maximum(0.6 - z + 0.1 * np.sin(0.3 * yy), 0.0).astype(np.float32)
+    rain = rng.uniform(1, 3, (ny, nx)).astype(np.float32) * 1e-5
+    infil = rng.uniform(0, 5, (ny, nx)).astype(np.float32) * 1e-6
+    count = ctypes.c_int32()
+    rbs.lib.call("rbs_hllc_variant_count", ctypes.byref(count))
+    outs = []
+    for v in range(count.value):
+        g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.035)
+        g.set_open_boundaries({"west": "open", "east": "fixed_stage"}, {"east": [(0.0, 0.55)]})
+        g.set_rain_field(rain)
+        g.set_infiltration(infil)
+        g.arm_source_compensation()
+        rbs.lib.call("rbs_set_hllc_variant", g._handle, v)
+        t = 0.0
+        for _ in range(40):
+            dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+            rbs.hl.step(g, dt, rain_scale=1.0, t=t)
+            t += dt
+        outs.append((g.h.numpy(), g.hu.numpy(), g.hv.numpy(), g.h_comp.numpy()))
+        g.close()
+    for v, o in enumerate(outs[1:], start=1):
+        for name, a, b in zip(("h", "hu", "hv", "h_comp"), outs[0], o):
+            assert np.array_equal(a, b), f"variant {v}: {name} differs"
