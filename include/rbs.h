@@ -245,9 +245,10 @@ int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is
  * kernels with HIP events on the state's stream; *avg_ms receives the mean.  Advances
  * the state. */
 int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
-/* The LI kernel variant a batch launches for all its members (the default unless
- * set); takes effect at the next advance. */
-int rbs_batch_set_variant(rbs_batch *b, int32_t variant);
+/* The LI kernel variants a batch launches: `variant` for members without the bedload
+ * integral, `variant_sed` for members with it (library defaults unless set); takes
+ * effect at the next advance. */
+int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed);
 /* Host-side cost of membership changes so far (table rebuilds and graph captures:
  * count and wall milliseconds), graph replays launched, and the device time of the
  * batched steps (HIP events around each advance). */

@@ -20,7 +20,8 @@ struct rbs_batch {
   int capacity = 0;
   std::vector<rbs_state *> slot;  // slot -> member (nullptr = free)
   hipStream_t stream = nullptr;
-  int variant = li_default_variant();
+  int variant = li_default_variant();          // members without the bedload integral
+  int variant_sed = li_default_sed_variant();  // members with it
   int chunk = 16;
   int plain_tiles = 0;   // tiles of members without the bedload integral (listed first)
   // device tables (rebuilt when membership changes)
@@ -83,10 +84,10 @@ static int rebuild(rbs_batch *b) {
   std::vector<InflowArgs> inflow;
   for (int i = 0; i < n; ++i) {
     rbs_state *s = b->slot[b->order[i]];
-    s->li_variant = b->variant;
+    const bool sed = (s->armed & RBS_ARM_SED) != 0;
+    s->li_variant = sed ? b->variant_sed : b->variant;
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
-    const bool sed = (s->armed & RBS_ARM_SED) != 0;
     std::vector<int> &seg = sed ? member_sed : member;
     first[i] = (int)seg.size();
     for (int t = 0; t < nt; ++t) seg.push_back(i);
@@ -169,7 +170,7 @@ static int capture(rbs_batch *b) {
       rc = li_batch_launch(b->variant, false, (const LiArgs *)b->d_args, b->d_tile_member,
                            b->d_tile_first, b->d_dims, grid_tiles, b->stream);
     if (rc == RBS_OK && sed)
-      rc = li_batch_launch(b->variant, true, (const LiArgs *)b->d_args, b->d_tile_member,
+      rc = li_batch_launch(b->variant_sed, true, (const LiArgs *)b->d_args, b->d_tile_member,
                            b->d_tile_first, b->d_dims, grid_tiles, b->stream);
     if (rc == RBS_OK)
       rc = batch_finalize_launch(b->d_fin, b->d_dims, b->capacity, b->d_status, b->stream);
@@ -387,14 +388,16 @@ int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
   return RBS_OK;
 }
 
-int rbs_batch_set_variant(rbs_batch *b, int32_t variant) {
+int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed) {
   if (!b) return set_error("null batch"), RBS_E_ARG;
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
-  if (variant < 0 || variant >= li_variant_count())
-    return set_error("LI variant %d is not in [0, %d)", variant, li_variant_count()), RBS_E_ARG;
-  if (variant != b->variant) {
+  for (int v : {variant, variant_sed})
+    if (v < 0 || v >= li_variant_count())
+      return set_error("LI variant %d is not in [0, %d)", v, li_variant_count()), RBS_E_ARG;
+  if (variant != b->variant || variant_sed != b->variant_sed) {
     b->variant = variant;
-    b->dirty = true;  // tables follow the new tile shape ...
+    b->variant_sed = variant_sed;
+    b->dirty = true;  // tables follow the new tile shapes ...
     drop_graph(b);    // ... and the graph launches the new kernels
   }
   return RBS_OK;

@@ -215,6 +215,7 @@ int hllc_default_variant();
 void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
+int li_default_sed_variant();
 void li_variant_shape(int v, int *bx, int *by, int *family);
 int ensure_partials(rbs_state *s, int nblk);
 
