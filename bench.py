@@ -117,6 +117,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--rate", type=float, default=4.0, help="served arrivals per second")
     ap.add_argument("--chunk", type=int, default=64, help="serve: batched steps per advance")
+    ap.add_argument("--sed-variant", type=int, default=-1,
+                    help="serve: batch LI variant for bedload members (-1: library default)")
     ap.add_argument("--capacity-sample", type=int, default=120,
                     help="serve: requests replayed as a burst and as plain launches")
     ap.add_argument("--sim-hours", type=float, default=1.0, help="simulated time per scenario")
@@ -407,11 +409,16 @@ def bench_serve(args) -> int:
     reqs = make_requests(args.requests, rate_per_s=args.rate, seed=20261206, pool=pool,
                          end_time=3600.0 * args.sim_hours)
     log(f"[serve] {len(reqs)} requests generated in {time.time() - t0:.0f} s")
-    srv = Server(capacity=256, chunk=args.chunk)
+    variants = None
+    if args.variant >= 0 or args.sed_variant >= 0:
+        vd, vs = default_batch_variants()
+        variants = (args.variant if args.variant >= 0 else vd,
+                    args.sed_variant if args.sed_variant >= 0 else vs)
+    srv = Server(capacity=256, chunk=args.chunk, variants=variants)
     stats = srv.serve(reqs, log=log)
     log(f"[serve] {stats}")
-    cap = (serve_capacity(reqs[: args.capacity_sample], args.chunk) if args.capacity_sample > 0
-           else None)
+    cap = (serve_capacity(reqs[: args.capacity_sample], args.chunk, variants)
+           if args.capacity_sample > 0 else None)
     kinds = {}
     for r in srv.results:
         kinds.setdefault(r.kind, []).append(r.time_to_result)
@@ -446,7 +453,18 @@ def bench_serve(args) -> int:
     return 0
 
 
-def serve_capacity(reqs, chunk: int) -> dict:
+def default_batch_variants() -> tuple[int, int]:
+    """The library's batch defaults: (members without, with the bedload integral)."""
+    from rbs.ensemble import Batch
+
+    b = Batch(1)
+    try:
+        return b.variants()
+    finally:
+        b.close()
+
+
+def serve_capacity(reqs, chunk: int, variants=None) -> dict:
     """Capacity of the server against plain per-request launches on the same requests:
     the server takes them as one burst (all arrive at once); plain runs them one after
     another through the reference's tick loop.  Every served result must equal its
@@ -456,7 +474,7 @@ def serve_capacity(reqs, chunk: int) -> dict:
     from rbs.serve import Server, plain
 
     burst = [dataclasses.replace(r, arrival_s=0.0) for r in reqs]
-    srv = Server(capacity=256, chunk=chunk, keep_results=True)
+    srv = Server(capacity=256, chunk=chunk, keep_results=True, variants=variants)
     st = srv.serve(burst, log=log)
     served = {r.rid: r for r in srv.results}
     t_plain, same = 0.0, True

@@ -249,6 +249,7 @@ int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
  * integral, `variant_sed` for members with it (library defaults unless set); takes
  * effect at the next advance. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed);
+int rbs_batch_get_variant(rbs_batch *b, int32_t *variant, int32_t *variant_sed);
 /* Host-side cost of membership changes so far (table rebuilds and graph captures:
  * count and wall milliseconds), graph replays launched, and the device time of the
  * batched steps (HIP events around each advance). */

@@ -388,6 +388,13 @@ int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
   return RBS_OK;
 }
 
+int rbs_batch_get_variant(rbs_batch *b, int32_t *variant, int32_t *variant_sed) {
+  if (!b || !variant || !variant_sed) return set_error("null argument"), RBS_E_ARG;
+  *variant = b->variant;
+  *variant_sed = b->variant_sed;
+  return RBS_OK;
+}
+
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed) {
   if (!b) return set_error("null batch"), RBS_E_ARG;
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;

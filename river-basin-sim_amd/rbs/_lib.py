@@ -104,6 +104,7 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_li_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_li_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32, _pi32]),
     "rbs_batch_set_variant": (ctypes.c_int, [_vp, _i32, _i32]),
+    "rbs_batch_get_variant": (ctypes.c_int, [_vp, _pi32, _pi32]),
     "rbs_set_hllc_variant": (ctypes.c_int, [_vp, _i32]),
     "rbs_hllc_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_hllc_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),

@@ -56,6 +56,15 @@ class Batch:
         _lib.call("rbs_batch_advance", self._handle, int(steps), ctypes.byref(n))
         return int(n.value)
 
+    def set_variants(self, variant: int, variant_sed: int) -> None:
+        """LI kernel variants for members without / with the bedload integral."""
+        _lib.call("rbs_batch_set_variant", self._handle, int(variant), int(variant_sed))
+
+    def variants(self) -> tuple[int, int]:
+        v, vs = ctypes.c_int32(), ctypes.c_int32()
+        _lib.call("rbs_batch_get_variant", self._handle, ctypes.byref(v), ctypes.byref(vs))
+        return int(v.value), int(vs.value)
+
     def launch(self, steps: int) -> None:
         """Enqueue up to ``steps`` batched steps and return at once (see collect)."""
         _lib.call("rbs_batch_launch", self._handle, int(steps))

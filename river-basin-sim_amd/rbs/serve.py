@@ -185,8 +185,11 @@ class Server:
     """Continuous-batching server over one GPU."""
 
     def __init__(self, *, capacity: int = 256, hbm_budget: int = int(0.85 * HBM_BYTES),
-                 chunk: int = 32, keep_results: bool = False, prep_threads: int = 2):
+                 chunk: int = 32, keep_results: bool = False, prep_threads: int = 2,
+                 variants: tuple[int, int] | None = None):
         self.batch = Batch(capacity)
+        if variants is not None:
+            self.batch.set_variants(*variants)
         self.capacity = capacity
         self.budget = hbm_budget
         self.chunk = chunk
