@@ -599,9 +599,9 @@ static const LiVariant &variant_at(int v) {
 
 int li_variant_count() { return kNumLiVariants; }
 int li_default_variant() { return kDefaultLiVariant; }
-// with the bedload integral the LDS tile keeps its occupancy (the strip's extra window
-// registers and per-face loads cost more): 64x8, measured fastest on reach shapes
-int li_default_sed_variant() { return 1; }
+// batches of bedload members: strips of 16 rows (measured fastest inside a batch,
+// where the members' tiles fill the GPU whatever their shapes)
+int li_default_sed_variant() { return 6; }
 void li_variant_shape(int v, int *bx, int *by, int *family) {
   const LiVariant &x = variant_at(v);
   *bx = x.bx;
