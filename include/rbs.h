@@ -60,8 +60,17 @@ enum rbs_field {
   RBS_F_DZ_UNAPPLIED = 17, /* (ny, nx) f64 bed change the bounds refused      */
   RBS_F_DZ_LO = 18,   /* (ny, nx)   f32  lower bound on cumulative change     */
   RBS_F_DZ_HI = 19,   /* (ny, nx)   f32  upper bound                          */
-  RBS_F_ETA = 20,     /* (ny, nx)   f32  water surface h + z (read-only)      */
-  RBS_F_COUNT = 21
+  RBS_F_ETA = 20,     /* (ny, nx)   f32  water surface (read-only): h + z, or
+                         the storage curve when channels are armed  */
+  /* sub-grid channels (channels.py:178-199) */
+  RBS_F_CH_W = 21,    /* (ny, nx)   f32  channel width, m (0: no channel)    */
+  RBS_F_CH_D = 22,    /* (ny, nx)   f32  bank-full depth below the bed, m    */
+  RBS_F_CH_N = 23,    /* (ny, nx)   f32  channel Manning n                   */
+  RBS_F_QX_CH = 24,   /* (ny, nx+1) f32  x-face discharge per channel width  */
+  RBS_F_QX_FP = 25,   /* (ny, nx+1) f32  x-face discharge per floodplain width */
+  RBS_F_QY_CH = 26,   /* (ny+1, nx) f32                                      */
+  RBS_F_QY_FP = 27,   /* (ny+1, nx) f32                                      */
+  RBS_F_COUNT = 28
 };
 
 /* ---- arming flags (state.py:98-171, hllc.py:123-152, sediment.py:658-709) -- */
@@ -70,6 +79,7 @@ enum rbs_field {
 #define RBS_ARM_INFIL (1u << 2)  /* infiltration field                      */
 #define RBS_ARM_HLLC (1u << 3)   /* momentum hu, hv + HLLC stage buffers    */
 #define RBS_ARM_SED (1u << 4)    /* d50, transport integrals, dz_cum ...    */
+#define RBS_ARM_CHAN (1u << 5)   /* sub-grid channel geometry + split faces */
 
 /* ---- edges and edge types (state.py:81-96) ------------------------------- */
 #define RBS_EDGE_WEST 0
@@ -129,8 +139,13 @@ int rbs_device_count(int32_t *count);
 int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state **out);
 int rbs_state_destroy(rbs_state *st);
 /* State.arm_source_compensation / set_infiltration / set_rain_field / arm_hllc /
- * arm_sediment (state.py:98-171, hllc.py:123, sediment.py:658).  Idempotent per
- * flag except RBS_ARM_SED, which refuses a second arming (sediment.py:680-684). */
+ * arm_sediment / arm_channels (state.py:98-171, hllc.py:123, sediment.py:658,
+ * channels.py:289-330).  Idempotent per flag except RBS_ARM_SED, which refuses a
+ * second arming (sediment.py:680-684).  RBS_ARM_CHAN allocates the geometry (width
+ * and depth 0, channel n a copy of n) and the face components (0); with it armed the
+ * local-inertial step, its CFL reduction and the bedload integral take their
+ * sub-grid forms (local_inertial.py:617-680, :487-500, sediment.py:276-377).  The
+ * HLLC scheme ignores channels, as the reference's does. */
 int rbs_arm(rbs_state *st, uint32_t arm);
 /* Armed flags of the state. */
 int rbs_armed(const rbs_state *st, uint32_t *arm);
@@ -237,6 +252,12 @@ int rbs_li_variant_count(int32_t *count);
  * with two columns per lane); *is_default = 1 for the library default. */
 int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *family,
                         int32_t *is_default);
+/* Tile variants of the LI step of a state armed with sub-grid channels (the LDS
+ * kernel that carries the two face components), as for LI; rbs_time_li_kernel times
+ * whichever applies to the state. */
+int rbs_set_chan_variant(rbs_state *st, int32_t variant);
+int rbs_chan_variant_count(int32_t *count);
+int rbs_chan_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default);
 /* HLLC stage-kernel tile variants (LDS per block against halo overhead), as for LI. */
 int rbs_set_hllc_variant(rbs_state *st, int32_t variant);
 int rbs_hllc_variant_count(int32_t *count);

@@ -51,6 +51,8 @@ class _OrcState(ctypes.Structure):
         ("dh", _fp), ("dhu", _fp), ("dhv", _fp), ("h1", _fp), ("hu1", _fp), ("hv1", _fp),
         ("d50", _fp), ("qs_int_x", _fp), ("qs_comp_x", _fp),
         ("qs_int_y", _fp), ("qs_comp_y", _fp),
+        ("ch_w", _fp), ("ch_d", _fp), ("ch_n", _fp),
+        ("qx_ch", _fp), ("qx_fp", _fp), ("qy_ch", _fp), ("qy_fp", _fp),
     ]  # fmt: skip
 
 
@@ -90,7 +92,12 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_rebuild_bed.argtypes = [ctypes.c_int32, ctypes.c_int32, _fp, _dp, _fp]
         L.orc_inject_point_sources_c.argtypes = [sp, ctypes.c_int32, _ip, _ip, _fp, _fp]
-        for fn in ("orc_pow73", "orc_cbrt", "orc_pow15"):
+        for fn in ("orc_eta_subgrid", "orc_column_depth"):
+            getattr(L, fn).argtypes = [ctypes.c_float] * (5 if fn == "orc_eta_subgrid" else 4)
+            getattr(L, fn).restype = ctypes.c_float
+        L.orc_manning_denominator_radius.argtypes = [ctypes.c_float] * 6
+        L.orc_manning_denominator_radius.restype = ctypes.c_float
+        for fn in ("orc_pow73", "orc_pow43", "orc_cbrt", "orc_pow15"):
             getattr(L, fn).argtypes = [ctypes.c_float]
             getattr(L, fn).restype = ctypes.c_float
         L.orc_set_pow_exact.argtypes = [ctypes.c_int]
@@ -155,6 +162,8 @@ class State:
         self._hl = None
         self.d50 = None
         self.qs_int_x = self.qs_comp_x = self.qs_int_y = self.qs_comp_y = None
+        self.ch_w = self.ch_d = self.ch_n = None
+        self.qx_ch = self.qx_fp = self.qy_ch = self.qy_fp = None
 
     # --- arming (state.py:98-171) ---
     def set_infiltration(self, infil):
@@ -207,6 +216,23 @@ class State:
         self.qs_int_y = np.zeros((ny + 1, nx), np.float32)
         self.qs_comp_y = np.zeros((ny + 1, nx), np.float32)
 
+    def arm_channels(self, width, depth, manning=None):
+        """channels.py:289-330 with validate_geometry's normalisation (:275-277):
+        a cell without both a width and a depth carries w = d = 0; the roughness
+        defaults to the floodplain's; the face components start at zero."""
+        ny, nx = self.ny, self.nx
+        w = _f32(width, (ny, nx))
+        d = _f32(depth, (ny, nx))
+        n = self.n.copy() if manning is None else _f32(manning, (ny, nx))
+        has = (w > 0) & (d > 0)
+        w[~has] = 0.0
+        d[~has] = 0.0
+        self.ch_w, self.ch_d, self.ch_n = w, d, n
+        self.qx_ch = np.zeros((ny, nx + 1), np.float32)
+        self.qx_fp = np.zeros((ny, nx + 1), np.float32)
+        self.qy_ch = np.zeros((ny + 1, nx), np.float32)
+        self.qy_fp = np.zeros((ny + 1, nx), np.float32)
+
     def loss_volume(self, cell_area: float) -> float:
         if self.loss_cum is None:
             return 0.0
@@ -220,7 +246,8 @@ class State:
         s = _OrcState()
         s.ny, s.nx, s.dx = self.ny, self.nx, self.dx
         for name in ("h", "z", "n", "qx", "qy", "eta", "beta", "h_comp", "rain", "infil",
-                     "hu", "hv", "d50", "qs_int_x", "qs_comp_x", "qs_int_y", "qs_comp_y"):
+                     "hu", "hv", "d50", "qs_int_x", "qs_comp_x", "qs_int_y", "qs_comp_y",
+                     "ch_w", "ch_d", "ch_n", "qx_ch", "qx_fp", "qy_ch", "qy_fp"):
             setattr(s, name, _ptr(getattr(self, name)))
         s.loss_cum = _ptr(self.loss_cum, _dp)
         for k, e in enumerate(EDGES):
@@ -310,6 +337,22 @@ def inject_point_sources_c(st: State, rows, cols, add_depth, comp):
 
 def pow73(x: float) -> float:
     return float(lib().orc_pow73(x))
+
+
+def pow43(x: float) -> float:
+    return float(lib().orc_pow43(x))
+
+
+def eta_subgrid(h: float, z: float, w: float, d: float, dx: float) -> float:
+    return float(lib().orc_eta_subgrid(h, z, w, d, dx))
+
+
+def column_depth(h: float, w: float, d: float, dx: float) -> float:
+    return float(lib().orc_column_depth(h, w, d, dx))
+
+
+def manning_denominator_radius(q, h_flow, radius, n, g, dt) -> float:
+    return float(lib().orc_manning_denominator_radius(q, h_flow, radius, n, g, dt))
 
 
 def cbrt(x: float) -> float:

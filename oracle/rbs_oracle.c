@@ -45,14 +45,14 @@ static inline float fmn(float a, float b) { return a < b ? a : b; }
 static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo), hi); }
 
 /*
- * Powers of a positive float32 (wp.pow in friction.py:117, sediment.py:168, :192).
+ * Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
  * Warp's wp.pow is a libm powf, accurate to about an ulp, whose last bit differs
- * between math libraries.  Both this oracle and the HIP kernels evaluate the three
+ * between math libraries.  Both this oracle and the HIP kernels evaluate the four
  * powers the solver needs by the same sequence of IEEE basic operations, so the two
  * agree bit for bit, and the result is the correctly rounded float32 power except in
  * vanishingly rare ties: x^(-1/3) from a bit-pattern seed refined by three float32
  * and one float64 Newton step, then the power formed in float64 and rounded once.
- * tests/test_oracle_pow.py checks all three against the exact power.
+ * tests/test_oracle_pow.py checks all four against the exact power.
  */
 static inline void inv_cbrt64(float x, double *xd, double *yd) {
   uint32_t i;
@@ -88,6 +88,12 @@ float orc_cbrt(float x) { /* x^(1/3) = x * x^(-2/3) */
   inv_cbrt64(x, &a, &b);
   return (float)(a * (b * b));
 }
+float orc_pow43(float x) { /* x^(4/3) = x^2 * x^(-2/3) */
+  if (!g_pow_exact) return powf(x, (float)(4.0 / 3.0));
+  double a, b;
+  inv_cbrt64(x, &a, &b);
+  return (float)((a * a) * (b * b));
+}
 float orc_pow15(float x) { /* x^(3/2) = x * sqrt(x) */
   if (!g_pow_exact) return powf(x, 1.5f);
   double a = (double)x;
@@ -100,12 +106,70 @@ static inline float manning_denominator(float q_prev, float h_flow, float n, flo
   return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
 }
 
+/* friction.py:35-57 manning_denominator_radius: D = 1 + g dt n^2 |q| / (h R^(4/3)) */
+float orc_manning_denominator_radius(float q_prev, float h_flow, float radius, float n,
+                                     float g, float dt) {
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / (h_flow * orc_pow43(radius));
+}
+
+/* channels.py:59-72 eta_subgrid: the storage curve h -> eta */
+float orc_eta_subgrid(float h, float z, float w, float d, float dx) {
+  float eta = h + z;
+  if (w > 0.0f && d > 0.0f) {
+    float h_bf = w * d / dx;
+    if (h <= h_bf)
+      eta = z - d + h * dx / w;
+    else
+      eta = z + (h - h_bf);
+  }
+  return eta;
+}
+
+/* channels.py:75-85 column_depth: eta - z_ch, the depth that sets the wave speed */
+float orc_column_depth(float h, float w, float d, float dx) {
+  float col = h;
+  if (w > 0.0f && d > 0.0f) {
+    float h_bf = w * d / dx;
+    if (h <= h_bf)
+      col = h * dx / w;
+    else
+      col = d + (h - h_bf);
+  }
+  return col;
+}
+
+/* local_inertial.py:286-312 _channel_flux, per unit channel width; the face section
+ * from channels.py:99-131 (min width, higher invert, bank-full saturation, A/P) */
+static inline float channel_flux(float q_prev, float eta_max, float slope, float w_face,
+                                 float z_bank, float z_ch, float n_ch, float dt, float g) {
+  if (w_face <= 0.0f) return 0.0f;
+  float h_ch = fmn(eta_max, z_bank) - z_ch;
+  if (h_ch < HDRY_F32) return 0.0f;
+  float radius = w_face * h_ch / (w_face + 2.0f * h_ch);
+  float num = q_prev - g * h_ch * dt * slope;
+  float den = orc_manning_denominator_radius(q_prev, h_ch, radius, n_ch, g, dt);
+  return num / den;
+}
+
+/* local_inertial.py:315-331 _floodplain_flux: the plain face update on the bank */
+static inline float floodplain_flux(float q_prev, float eta_max, float slope, float z_bank,
+                                    float n_fp, float dt, float g);
+
 /* sources.py:274-290 kahan_add -> (total, comp) */
 static inline void kahan_add(float *total, float *comp, float inc) {
   float y = inc - *comp;
   float t = *total + y;
   *comp = (t - *total) - y;
   *total = t;
+}
+
+static inline float floodplain_flux(float q_prev, float eta_max, float slope, float z_bank,
+                                    float n_fp, float dt, float g) {
+  float h_fp = eta_max - z_bank;
+  if (h_fp < HDRY_F32) return 0.0f;
+  float num = q_prev - g * h_fp * dt * slope;
+  float den = manning_denominator(q_prev, h_fp, n_fp, g, dt);
+  return num / den;
 }
 
 /* ========================================================================== */
@@ -118,6 +182,94 @@ static void li_compute_eta(orc_state *s) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ny; ++i)
     for (int j = 0; j < nx; ++j) s->eta[IX(i, j, nx)] = s->h[IX(i, j, nx)] + s->z[IX(i, j, nx)];
+}
+
+/* local_inertial.py:274-283 compute_eta_channels */
+static void li_compute_eta_channels(orc_state *s) {
+  const int ny = s->ny, nx = s->nx;
+  const float dx = (float)s->dx;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ny; ++i)
+    for (int j = 0; j < nx; ++j) {
+      const size_t k = IX(i, j, nx);
+      s->eta[k] = orc_eta_subgrid(s->h[k], s->z[k], s->ch_w[k], s->ch_d[k], dx);
+    }
+}
+
+/* local_inertial.py:334-381 update_qx_channels / :384-429 update_qy_channels: one
+ * interior face between cells a (left/top) and b (right/bottom); writes both
+ * components and the total per unit cell width. */
+static inline void channel_face(const orc_state *s, size_t a, size_t b, float dx, float dt,
+                                float g, float *q, float *q_ch, float *q_fp) {
+  float eta_a = s->eta[a], eta_b = s->eta[b];
+  float eta_max = fmx(eta_a, eta_b);
+  float slope = (eta_b - eta_a) / dx;
+  float z_bank = fmx(s->z[a], s->z[b]);
+  float fp = floodplain_flux(*q_fp, eta_max, slope, z_bank, 0.5f * (s->n[a] + s->n[b]), dt, g);
+  float w_face = fmn(s->ch_w[a], s->ch_w[b]);
+  float z_ch = fmx(s->z[a] - s->ch_d[a], s->z[b] - s->ch_d[b]);
+  float ch = channel_flux(*q_ch, eta_max, slope, w_face, z_bank, z_ch,
+                          0.5f * (s->ch_n[a] + s->ch_n[b]), dt, g);
+  *q_fp = fp;
+  *q_ch = ch;
+  float frac = w_face / dx;
+  *q = fp * (1.0f - frac) + ch * frac;
+}
+
+static void li_update_faces_channels(orc_state *s, float dx, float dt, float g) {
+  const int ny = s->ny, nx = s->nx, w = nx + 1;
+  if (nx > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ny; ++i)
+      for (int j = 1; j < nx; ++j) {
+        const size_t f = IX(i, j, w);
+        channel_face(s, IX(i, j - 1, nx), IX(i, j, nx), dx, dt, g, &s->qx[f], &s->qx_ch[f],
+                     &s->qx_fp[f]);
+      }
+  }
+  if (ny > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < ny; ++i)
+      for (int j = 0; j < nx; ++j) {
+        const size_t f = IX(i, j, nx);
+        channel_face(s, IX(i - 1, j, nx), IX(i, j, nx), dx, dt, g, &s->qy[f], &s->qy_ch[f],
+                     &s->qy_fp[f]);
+      }
+  }
+}
+
+/* local_inertial.py:432-484 limit_qx_channels / limit_qy_channels: the total's donor
+ * factor scales the total and both components (a zero total is left alone). */
+static void li_limit_channels(orc_state *s) {
+  const int ny = s->ny, nx = s->nx, w = nx + 1;
+  if (nx > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ny; ++i)
+      for (int j = 1; j < nx; ++j) {
+        const size_t f = IX(i, j, w);
+        float q = s->qx[f];
+        if (q == 0.0f) continue;
+        float b = s->beta[IX(i, j - 1, nx)];
+        if (q < 0.0f) b = s->beta[IX(i, j, nx)];
+        s->qx[f] = q * b;
+        s->qx_ch[f] = s->qx_ch[f] * b;
+        s->qx_fp[f] = s->qx_fp[f] * b;
+      }
+  }
+  if (ny > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < ny; ++i)
+      for (int j = 0; j < nx; ++j) {
+        const size_t f = IX(i, j, nx);
+        float q = s->qy[f];
+        if (q == 0.0f) continue;
+        float b = s->beta[IX(i - 1, j, nx)];
+        if (q < 0.0f) b = s->beta[IX(i, j, nx)];
+        s->qy[f] = q * b;
+        s->qy_ch[f] = s->qy_ch[f] * b;
+        s->qy_fp[f] = s->qy_fp[f] * b;
+      }
+  }
 }
 
 /* local_inertial.py:50-58 face_h_flow + :72-99 update_qx (interior faces j=1..nx-1) */
@@ -328,6 +480,12 @@ static void li_open_outflow(orc_state *s, double dt) {
 float orc_li_reduce_hmax(const orc_state *s) {
   const size_t cells = (size_t)s->ny * s->nx;
   float m = 0.0f;
+  if (s->ch_w) { /* local_inertial.py:487-500 reduce_column_max */
+    const float dx = (float)s->dx;
+    for (size_t k = 0; k < cells; ++k)
+      m = fmx(m, orc_column_depth(s->h[k], s->ch_w[k], s->ch_d[k], dx));
+    return m;
+  }
   for (size_t k = 0; k < cells; ++k) m = fmx(m, s->h[k]);
   return m;
 }
@@ -340,17 +498,28 @@ double orc_li_compute_dt(const orc_state *s, double alpha, double dt_max) {
   return dt < dt_max ? dt : dt_max;
 }
 
-/* local_inertial.py:547-727 step (no sub-grid channels) */
+/* local_inertial.py:547-727 step; sub-grid channels switch the eta, face and
+ * limiter passes onto their channel forms (:617-680) */
 void orc_li_step(orc_state *s, double dt, double rain, int limit, double rain_scale) {
   const float dxf = (float)s->dx, dtf = (float)dt, gf = G_F32;
-  li_compute_eta(s);
-  if (s->nx > 1) li_update_qx(s, dxf, dtf, gf);
-  if (s->ny > 1) li_update_qy(s, dxf, dtf, gf);
+  const int chan = s->ch_w != NULL;
+  if (chan) {
+    li_compute_eta_channels(s);
+    li_update_faces_channels(s, dxf, dtf, gf);
+  } else {
+    li_compute_eta(s);
+    if (s->nx > 1) li_update_qx(s, dxf, dtf, gf);
+    if (s->ny > 1) li_update_qy(s, dxf, dtf, gf);
+  }
   li_closed_bc(s);
   if (limit) {
     li_outflow_beta(s, dxf, dtf);
-    if (s->nx > 1) li_limit_qx(s);
-    if (s->ny > 1) li_limit_qy(s);
+    if (chan) {
+      li_limit_channels(s);
+    } else {
+      if (s->nx > 1) li_limit_qx(s);
+      if (s->ny > 1) li_limit_qy(s);
+    }
   }
   if (s->d50) orc_accumulate_transport(s, dt);
   const int compensated = s->h_comp != NULL;
@@ -401,10 +570,63 @@ float orc_face_capacity(float q, float h, float radius, float n, float d50) {
   return q < 0.0f ? -qs : qs;
 }
 
+/* sediment.py:276-302 _combined_capacity: floodplain and channel components, each
+ * from its own flow depth and hydraulic radius, mixed by the face's width fraction */
+static inline float combined_capacity(float q_ch, float q_fp, float eta_max, float z_bank,
+                                      float z_ch, float w_face, float n_fp, float n_ch,
+                                      float d50, float dx) {
+  float h_fp = eta_max - z_bank;
+  float qs_fp = orc_face_capacity(q_fp, h_fp, h_fp, n_fp, d50);
+  float qs_ch = 0.0f;
+  if (w_face > 0.0f) {
+    float h_ch = fmn(eta_max, z_bank) - z_ch;
+    if (h_ch >= HDRY_F32)
+      qs_ch = orc_face_capacity(q_ch, h_ch, w_face * h_ch / (w_face + 2.0f * h_ch), n_ch, d50);
+  }
+  float frac = w_face / dx;
+  return qs_fp * (1.0f - frac) + qs_ch * frac;
+}
+
+/* sediment.py:305-377 accumulate_qs_x_channels / accumulate_qs_y_channels */
+static void accumulate_transport_channels(orc_state *s, float dtf) {
+  const int ny = s->ny, nx = s->nx, w = nx + 1;
+  const float dx = (float)s->dx;
+  if (nx > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ny; ++i)
+      for (int j = 1; j < nx; ++j) {
+        const size_t a = IX(i, j - 1, nx), b = IX(i, j, nx), f = IX(i, j, w);
+        float qs = combined_capacity(
+            s->qx_ch[f], s->qx_fp[f], fmx(s->eta[a], s->eta[b]), fmx(s->z[a], s->z[b]),
+            fmx(s->z[a] - s->ch_d[a], s->z[b] - s->ch_d[b]), fmn(s->ch_w[a], s->ch_w[b]),
+            0.5f * (s->n[a] + s->n[b]), 0.5f * (s->ch_n[a] + s->ch_n[b]),
+            0.5f * (s->d50[a] + s->d50[b]), dx);
+        kahan_add(&s->qs_int_x[f], &s->qs_comp_x[f], qs * dtf);
+      }
+  }
+  if (ny > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < ny; ++i)
+      for (int j = 0; j < nx; ++j) {
+        const size_t a = IX(i - 1, j, nx), b = IX(i, j, nx), f = IX(i, j, nx);
+        float qs = combined_capacity(
+            s->qy_ch[f], s->qy_fp[f], fmx(s->eta[a], s->eta[b]), fmx(s->z[a], s->z[b]),
+            fmx(s->z[a] - s->ch_d[a], s->z[b] - s->ch_d[b]), fmn(s->ch_w[a], s->ch_w[b]),
+            0.5f * (s->n[a] + s->n[b]), 0.5f * (s->ch_n[a] + s->ch_n[b]),
+            0.5f * (s->d50[a] + s->d50[b]), dx);
+        kahan_add(&s->qs_int_y[f], &s->qs_comp_y[f], qs * dtf);
+      }
+  }
+}
+
 /* sediment.py:218-273 accumulate_qs_x / accumulate_qs_y; :444-488 dispatch */
 void orc_accumulate_transport(orc_state *s, double dt) {
   const int ny = s->ny, nx = s->nx, w = nx + 1;
   const float dtf = (float)dt;
+  if (s->ch_w) {
+    accumulate_transport_channels(s, dtf);
+    return;
+  }
   if (nx > 1) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < ny; ++i)

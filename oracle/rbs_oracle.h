@@ -49,10 +49,15 @@ typedef struct orc_state {
   float *d50;
   float *qs_int_x, *qs_comp_x;   /* (ny, nx+1) */
   float *qs_int_y, *qs_comp_y;   /* (ny+1, nx) */
+  /* sub-grid channels (channels.py:178-199); NULL unless armed */
+  float *ch_w, *ch_d, *ch_n;      /* (ny, nx) width, bank-full depth, Manning n */
+  float *qx_ch, *qx_fp;           /* (ny, nx+1) channel / floodplain components */
+  float *qy_ch, *qy_fp;           /* (ny+1, nx) */
 } orc_state;
 
-/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(1/3), x^(3/2). */
+/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(4/3), x^(1/3), x^(3/2). */
 float orc_pow73(float x);
+float orc_pow43(float x);
 float orc_cbrt(float x);
 float orc_pow15(float x);
 /* 1 (default): correctly rounded powers, bit-identical to the HIP kernels;
@@ -69,6 +74,13 @@ double orc_li_compute_dt(const orc_state *s, double alpha, double dt_max);
 void orc_li_step(orc_state *s, double dt, double rain, int limit, double rain_scale);
 /* A 1000-step style driver: n_steps fixed-dt steps (used by the CPU baseline). */
 void orc_li_run_fixed(orc_state *s, double dt, int n_steps, double rain, double rain_scale);
+
+/* ---- sub-grid channels (channels.py:59-138) ------------------------------ */
+float orc_eta_subgrid(float h, float z, float w, float d, float dx);
+float orc_column_depth(float h, float w, float d, float dx);
+/* friction.py:35-57 manning_denominator_radius */
+float orc_manning_denominator_radius(float q_prev, float h_flow, float radius, float n,
+                                     float g, float dt);
 
 /* ---- HLLC ------------------------------------------------------------------ */
 float orc_hllc_reduce_wavespeed(const orc_state *s);

@@ -252,6 +252,9 @@ int rbs_batch_add(rbs_batch *b, rbs_state *st, const rbs_run_params *p, double t
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   if (p->scheme != RBS_SCHEME_LI) return set_error("a batch advances local-inertial scenarios"), RBS_E_ARG;
   if (!p->limit) return set_error("batched steps always run the donor limiter"), RBS_E_ARG;
+  if (st->armed & RBS_ARM_CHAN)
+    return set_error("states with sub-grid channels run on their own (rbs_run_until), not in a "
+                     "batch"), RBS_E_STATE;
   for (int k = 0; k < b->capacity; ++k)
     if (b->slot[k] == st) return set_error("state is already in the batch"), RBS_E_STATE;
   int k = 0;

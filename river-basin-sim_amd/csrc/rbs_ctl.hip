@@ -11,12 +11,17 @@
 namespace rbs {
 
 // Maximum-only reduction of the CFL speed of the current state, into partials.
-// LI: h (local_inertial.py:503-512); HLLC: max(|u|,|v|) + sqrt(g h) (hllc.py:948-966).
-// Also sums f64(h) so a prime leaves the ledger volume current.
+// LI: h (local_inertial.py:503-512), or the water-column depth when the state has
+// sub-grid channels (cw != null, :487-500); HLLC: max(|u|,|v|) + sqrt(g h)
+// (hllc.py:948-966).  Also sums f64(h) so a prime leaves the ledger volume current.
 __global__ __launch_bounds__(kThreads) void reduce_speed_kernel(const float *h, const float *hu,
-                                                                const float *hv, int64_t pitch,
-                                                                int ny, int nx, int scheme,
-                                                                Partials part, int rows_per_blk) {
+                                                                const float *hv, const float *cw,
+                                                                const float *cd, double dx64,
+                                                                int64_t pitch, int ny, int nx,
+                                                                int scheme, Partials part,
+                                                                int rows_per_blk) {
+  const float dx = (float)dx64;
+  const float rdx = 1.0f / dx;
   __shared__ double s_red[4 * (kThreads / 64)];
   const int r0 = blockIdx.x * rows_per_blk;
   float m = 0.0f;
@@ -27,7 +32,7 @@ __global__ __launch_bounds__(kThreads) void reduce_speed_kernel(const float *h, 
       const float hh = h[o];
       vol += (double)hh;
       if (scheme == RBS_SCHEME_LI) {
-        m = fmx(m, hh);
+        m = fmx(m, cw ? column_depth_u(hh, cw[o], cd[o], dx, rdx) : hh);
       } else {
         const float cc = sqrtf(kG * fmx(hh, 0.0f));
         float sp = cc;
@@ -253,8 +258,10 @@ int reduce_speed_launch(rbs_state *s, int scheme, int mode) {
     return RBS_E_STATE;
   }
   const float *h = s->h[s->cur];
+  const bool column = scheme == RBS_SCHEME_LI && (s->armed & RBS_ARM_CHAN);
   hipLaunchKernelGGL(reduce_speed_kernel, dim3(nblk), dim3(kThreads), 0, s->stream, h, s->hu,
-                     s->hv, s->pitch, s->ny, s->nx, scheme, s->part, rpb);
+                     s->hv, column ? s->ch_w : nullptr, column ? s->ch_d : nullptr, s->dx,
+                     s->pitch, s->ny, s->nx, scheme, s->part, rpb);
   RBS_HIP(hipGetLastError());
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinThreads), 0, s->stream, s->ctl, s->part,
                      nblk, s->dx * s->dx, (double)s->ny * (double)s->nx, s->d_sync, s->dx,

@@ -15,9 +15,9 @@ __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return fm
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Powers of a positive float32 (wp.pow in friction.py:117, sediment.py:168, :192).
+// Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
 // The device library's powf and the host's differ in the last bit, and at wet/dry
-// switches that difference grows; so the solver evaluates its three powers by one
+// switches that difference grows; so the solver evaluates its four powers by one
 // fixed sequence of IEEE basic operations (no FMA: the build sets
 // -ffp-contract=off), bit-identical on every IEEE machine and correctly rounded but
 // for vanishingly rare ties: x^(-1/3) from a bit-pattern seed, three float32 and one
@@ -46,6 +46,11 @@ __device__ __forceinline__ float cbrt_f(float x) {  // x^(1/3) = x * x^(-2/3)
   double a, b;
   inv_cbrt64(x, a, b);
   return (float)(a * (b * b));
+}
+__device__ __forceinline__ float pow43(float x) {  // x^(4/3) = x^2 * x^(-2/3)
+  double a, b;
+  inv_cbrt64(x, a, b);
+  return (float)((a * a) * (b * b));
 }
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   const double a = (double)x;
@@ -103,6 +108,91 @@ __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, floa
   return num / den;
 }
 
+// ---- sub-grid channels (channels.py:59-138, local_inertial.py:266-500) -------------
+// The storage curve h -> eta (channels.py:59-72), in float32 IEEE operations; the
+// device forms divide by the uniform dx with div_uniform (the same correctly rounded
+// quotient), so host and device agree bit for bit.
+__host__ __device__ inline float eta_subgrid(float h, float z, float w, float d, float dx) {
+  float eta = h + z;
+  if (w > 0.0f && d > 0.0f) {
+    const float h_bf = w * d / dx;
+    if (h <= h_bf) eta = z - d + h * dx / w;
+    else eta = z + (h - h_bf);
+  }
+  return eta;
+}
+__device__ __forceinline__ float eta_subgrid_u(float h, float z, float w, float d, float dx,
+                                               float rdx) {
+  float eta = h + z;
+  if (w > 0.0f && d > 0.0f) {
+    const float h_bf = div_uniform(w * d, dx, rdx);
+    if (h <= h_bf) eta = z - d + h * dx / w;
+    else eta = z + (h - h_bf);
+  }
+  return eta;
+}
+// channels.py:75-85 column_depth: eta - z_ch, the depth the CFL bound sees.
+__device__ __forceinline__ float column_depth_u(float h, float w, float d, float dx, float rdx) {
+  float col = h;
+  if (w > 0.0f && d > 0.0f) {
+    const float h_bf = div_uniform(w * d, dx, rdx);
+    if (h <= h_bf) col = h * dx / w;
+    else col = d + (h - h_bf);
+  }
+  return col;
+}
+
+// One interior face of a channelled state (update_qx_channels :334-381 /
+// update_qy_channels :384-429): a = left/top cell, b = right/bottom cell; the
+// floodplain component (_floodplain_flux :315-331) and the channel component
+// (_channel_flux :286-312, friction.py:35-57) from their previous values, both
+// written back; returns the total per unit cell width.
+struct ChanCell {
+  float eta, z, n, w, d, nc;
+};
+__device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b, float &q_ch,
+                                           float &q_fp, float dx, float rdx, float dt) {
+  const float g = kG;
+  const float eta_max = fmx(a.eta, b.eta);
+  const float slope = div_uniform(b.eta - a.eta, dx, rdx);
+  const float z_bank = fmx(a.z, b.z);
+  float fp = 0.0f;
+  const float h_fp = eta_max - z_bank;
+  if (!(h_fp < kHDry)) {
+    const float num = q_fp - g * h_fp * dt * slope;
+    fp = num / manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt);
+  }
+  const float w_face = fmn(a.w, b.w);
+  float ch = 0.0f;
+  if (!(w_face <= 0.0f)) {
+    const float z_ch = fmx(a.z - a.d, b.z - b.d);
+    const float h_ch = fmn(eta_max, z_bank) - z_ch;
+    if (!(h_ch < kHDry)) {
+      const float radius = w_face * h_ch / (w_face + 2.0f * h_ch);
+      const float nc = 0.5f * (a.nc + b.nc);
+      const float num = q_ch - g * h_ch * dt * slope;
+      const float den = 1.0f + g * dt * nc * nc * fabsf(q_ch) / (h_ch * pow43(radius));
+      ch = num / den;
+    }
+  }
+  q_fp = fp;
+  q_ch = ch;
+  const float frac = div_uniform(w_face, dx, rdx);
+  return fp * (1.0f - frac) + ch * frac;
+}
+
+// limit_qx_channels / limit_qy_channels (:432-484) for one face: a non-zero total
+// takes its donor's factor (lo: the left/top cell's, for positive flow; hi: the
+// right/bottom cell's, for negative), and so do both components.  `q` is the
+// unlimited total.
+__device__ __forceinline__ void limit_components(float q, float &q_ch, float &q_fp, float b_lo,
+                                                 float b_hi) {
+  if (q == 0.0f) return;
+  const float b = q < 0.0f ? b_hi : b_lo;
+  q_ch = q_ch * b;
+  q_fp = q_fp * b;
+}
+
 // boundaries.py:94-106 _drain
 __device__ __forceinline__ float drain(float h_edge, float q_out, float dxdt) {
   float removed = dxdt * q_out;
@@ -121,6 +211,27 @@ __device__ __forceinline__ float face_capacity(float q, float h, float radius, f
   float excess = theta - th_c;
   float qs = 8.0f * pow15(excess) * sqrtf(sg * kG * d50 * d50 * d50);
   return q < 0.0f ? -qs : qs;
+}
+
+// sediment.py:276-302 _combined_capacity: the floodplain and channel components of a
+// face's bedload flux, each from its own depth and hydraulic radius, mixed by the
+// face's width fraction (per unit cell width).
+__device__ __forceinline__ float combined_capacity(float q_ch, float q_fp, const ChanCell &a,
+                                                   const ChanCell &b, float d50, float dx,
+                                                   float rdx) {
+  const float eta_max = fmx(a.eta, b.eta), z_bank = fmx(a.z, b.z);
+  const float h_fp = eta_max - z_bank;
+  const float qs_fp = face_capacity(q_fp, h_fp, h_fp, 0.5f * (a.n + b.n), d50);
+  const float w_face = fmn(a.w, b.w);
+  float qs_ch = 0.0f;
+  if (w_face > 0.0f) {
+    const float h_ch = fmn(eta_max, z_bank) - fmx(a.z - a.d, b.z - b.d);
+    if (h_ch >= kHDry)
+      qs_ch = face_capacity(q_ch, h_ch, w_face * h_ch / (w_face + 2.0f * h_ch),
+                            0.5f * (a.nc + b.nc), d50);
+  }
+  const float frac = div_uniform(w_face, dx, rdx);
+  return qs_fp * (1.0f - frac) + qs_ch * frac;
 }
 
 // Map a launch-order block id to a tile id so that the tiles one XCD executes are

@@ -171,6 +171,11 @@ struct rbs_state {
         *qs_comp_y = nullptr, *z0 = nullptr, *dz_lo = nullptr, *dz_hi = nullptr;
   double *dz_cum = nullptr, *dz_unapplied = nullptr;
   double *sed_part = nullptr;  // bed-change totals: 5 per block + the 5 results
+  // sub-grid channels: geometry, and the face components (ping-pong like qx / qy)
+  float *ch_w = nullptr, *ch_d = nullptr, *ch_n = nullptr;
+  float *qx_ch[2] = {nullptr, nullptr}, *qx_fp[2] = {nullptr, nullptr};
+  float *qy_ch[2] = {nullptr, nullptr}, *qy_fp[2] = {nullptr, nullptr};
+  int32_t chan_variant = 0;
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;
@@ -215,6 +220,10 @@ int hllc_default_variant();
 void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
+int chan_variant_count();
+int chan_default_variant();
+void chan_variant_shape(int v, int *bx, int *by);
+int chan_num_blocks(const rbs_state *s);
 int li_default_sed_variant();
 void li_variant_shape(int v, int *bx, int *by, int *family);
 int ensure_partials(rbs_state *s, int nblk);

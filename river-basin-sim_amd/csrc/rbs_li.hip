@@ -36,6 +36,7 @@ enum : int {
   LI_OPEN_E = 1 << 6,
   LI_OPEN_N = 1 << 7,
   LI_OPEN_S = 1 << 8,
+  LI_CHAN = 1 << 9,
 };
 
 struct LiArgs {
@@ -49,6 +50,9 @@ struct LiArgs {
   const float *infil;
   const float *d50;
   float *qs_int_x, *qs_comp_x, *qs_int_y, *qs_comp_y;
+  // sub-grid channels: geometry and the ping-pong face components
+  const float *ch_w, *ch_d, *ch_n;
+  float *qx_ch[2], *qx_fp[2], *qy_ch[2], *qy_fp[2];
   const DevCtl *ctl;
   Partials part;
   double dx64;
@@ -519,6 +523,222 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   write_partials(a, tile, sum, s_red);
 }
 
+// ---- family 3: LDS tile of a state with sub-grid channels --------------------------
+// The same phases as li_tile with the channel forms of three of them
+// (local_inertial.py:617-680): eta through the storage curve, each face updated as a
+// floodplain and a channel component recombined into the total, and the limiter
+// scaling the total and both components by the total's donor factor.  Continuity,
+// sources and sinks read the totals exactly as in the plain step; the bedload
+// integral mixes the two components' capacities (sediment.py:276-377); the CFL
+// partial is the maximum water-column depth (:487-500).
+template <int BX, int BY>
+__device__ __forceinline__ void li_chan_tile(const LiArgs &a, const int tile) {
+  const DevCtl *ctl = a.ctl;
+  if (ctl->active != 1) return;
+  const int cur = ctl->cur;
+  const float *__restrict__ h_in = a.h[cur];
+  float *__restrict__ h_out = a.h[cur ^ 1];
+  const float *__restrict__ qxc_in = a.qx_ch[cur];
+  const float *__restrict__ qxf_in = a.qx_fp[cur];
+  const float *__restrict__ qyc_in = a.qy_ch[cur];
+  const float *__restrict__ qyf_in = a.qy_fp[cur];
+  float *__restrict__ qx_out = a.qx[cur ^ 1];
+  float *__restrict__ qy_out = a.qy[cur ^ 1];
+  float *__restrict__ qxc_out = a.qx_ch[cur ^ 1];
+  float *__restrict__ qxf_out = a.qx_fp[cur ^ 1];
+  float *__restrict__ qyc_out = a.qy_ch[cur ^ 1];
+  float *__restrict__ qyf_out = a.qy_fp[cur ^ 1];
+
+  constexpr int EW = BX + 4, EH = BY + 4;  // cells two out
+  constexpr int RW = BX + 2, RH = BY + 2;  // cells one out
+  constexpr int QXW = BX + 3, QYH = BY + 3;
+  __shared__ float s_eta[EH][EW], s_z[EH][EW], s_n[EH][EW];
+  __shared__ float s_w[EH][EW], s_d[EH][EW], s_nc[EH][EW];
+  __shared__ float s_h[RH][RW], s_beta[RH][RW];
+  __shared__ float s_qx[RH][QXW], s_qxc[RH][QXW], s_qxf[RH][QXW];
+  __shared__ float s_qy[QYH][RW], s_qyc[QYH][RW], s_qyf[QYH][RW];
+  __shared__ double s_red[4 * (kThreads / 64)];
+
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int y0 = ty * BY, x0 = tx * BX;
+  const int ny = a.ny, nx = a.nx;
+  const int64_t P = a.pitch;
+  const int tid = threadIdx.x;
+  const int flags = a.flags;
+  const StepScalars k = step_scalars(a, ctl);
+  const float dx = (float)a.dx64;
+  const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
+  auto cell = [&](int lr, int lc) {
+    return ChanCell{s_eta[lr][lc], s_z[lr][lc], s_n[lr][lc], s_w[lr][lc], s_d[lr][lc],
+                    s_nc[lr][lc]};
+  };
+
+  // ---- phase A: stage the halo'd tile; eta through the storage curve -------------
+#pragma unroll
+  for (int k0 = 0; k0 < EH * EW; k0 += kThreads) {
+    const int kk = k0 + tid;
+    if (kk < EH * EW) {
+      const int lr = kk / EW, lc = kk - (kk / EW) * EW;
+      const int gi = clampi(y0 + lr - 2, 0, ny - 1), gj = clampi(x0 + lc - 2, 0, nx - 1);
+      const int64_t o = (int64_t)gi * P + gj;
+      const float hv = h_in[o], zv = a.z[o], wv = a.ch_w[o], dv = a.ch_d[o];
+      s_eta[lr][lc] = eta_subgrid_u(hv, zv, wv, dv, dx, rdx);  // compute_eta_channels
+      s_z[lr][lc] = zv;
+      s_n[lr][lc] = a.n[o];
+      s_w[lr][lc] = wv;
+      s_d[lr][lc] = dv;
+      s_nc[lr][lc] = a.ch_n[o];
+      if (lr >= 1 && lr <= BY + 2 && lc >= 1 && lc <= BX + 2) s_h[lr - 1][lc - 1] = hv;
+    }
+  }
+#pragma unroll
+  for (int k0 = 0; k0 < RH * QXW; k0 += kThreads) {
+    const int kk = k0 + tid;
+    if (kk < RH * QXW) {
+      const int lr = kk / QXW, lf = kk - (kk / QXW) * QXW;
+      const int gi = clampi(y0 + lr - 1, 0, ny - 1), gf = clampi(x0 + lf - 1, 0, nx);
+      const int64_t o = (int64_t)gi * P + gf;
+      s_qxc[lr][lf] = qxc_in[o];
+      s_qxf[lr][lf] = qxf_in[o];
+    }
+  }
+#pragma unroll
+  for (int k0 = 0; k0 < QYH * RW; k0 += kThreads) {
+    const int kk = k0 + tid;
+    if (kk < QYH * RW) {
+      const int lf = kk / RW, lc = kk - (kk / RW) * RW;
+      const int gf = clampi(y0 + lf - 1, 0, ny), gj = clampi(x0 + lc - 1, 0, nx - 1);
+      const int64_t o = (int64_t)gf * P + gj;
+      s_qyc[lf][lc] = qyc_in[o];
+      s_qyf[lf][lc] = qyf_in[o];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase B: two-component update of every face of the ring-1 cells ----------
+  // Boundary and out-of-domain faces (and their components) are 0.
+#pragma unroll
+  for (int k0 = 0; k0 < RH * QXW; k0 += kThreads) {
+    const int kk = k0 + tid;
+    if (kk < RH * QXW) {
+      const int lr = kk / QXW, lf = kk - (kk / QXW) * QXW;
+      const int r = y0 + lr - 1, f = x0 + lf - 1;
+      float q = 0.0f, qc = 0.0f, qf = 0.0f;
+      if (r >= 0 && r < ny && f >= 1 && f <= nx - 1) {
+        qc = s_qxc[lr][lf];
+        qf = s_qxf[lr][lf];
+        q = chan_face(cell(lr + 1, lf), cell(lr + 1, lf + 1), qc, qf, dx, rdx, k.dt);
+      }
+      s_qx[lr][lf] = q;
+      s_qxc[lr][lf] = qc;
+      s_qxf[lr][lf] = qf;
+    }
+  }
+#pragma unroll
+  for (int k0 = 0; k0 < QYH * RW; k0 += kThreads) {
+    const int kk = k0 + tid;
+    if (kk < QYH * RW) {
+      const int lf = kk / RW, lc = kk - (kk / RW) * RW;
+      const int f = y0 + lf - 1, c = x0 + lc - 1;
+      float q = 0.0f, qc = 0.0f, qf = 0.0f;
+      if (c >= 0 && c < nx && f >= 1 && f <= ny - 1) {
+        qc = s_qyc[lf][lc];
+        qf = s_qyf[lf][lc];
+        q = chan_face(cell(lf, lc + 1), cell(lf + 1, lc + 1), qc, qf, dx, rdx, k.dt);
+      }
+      s_qy[lf][lc] = q;
+      s_qyc[lf][lc] = qc;
+      s_qyf[lf][lc] = qf;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase C: donor factors from the totals ---------------------------------------
+  const bool limit = flags & LI_LIMIT;
+  if (limit) {
+#pragma unroll
+    for (int k0 = 0; k0 < RH * RW; k0 += kThreads) {
+      const int kk = k0 + tid;
+      if (kk < RH * RW) {
+        const int lr = kk / RW, lc = kk - (kk / RW) * RW;
+        s_beta[lr][lc] = donor_beta(s_qx[lr][lc], s_qx[lr][lc + 1], s_qy[lr][lc],
+                                    s_qy[lr + 1][lc], s_h[lr][lc], k.dtdx);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- phase D: limited faces and components, bedload, continuity, sources ---------
+  CellSums sum;
+  float m_col = 0.0f;  // reduce_column_max starts at 0
+#pragma unroll
+  for (int k0 = 0; k0 < BY * BX; k0 += kThreads) {
+    const int kk = k0 + tid;
+    const int lr = kk / BX, lc = kk - (kk / BX) * BX;
+    const int r = y0 + lr, c = x0 + lc;
+    if (kk < BY * BX && r < ny && c < nx) {
+      float qxL = s_qx[lr + 1][lc + 1], qxR = s_qx[lr + 1][lc + 2];
+      float qyT = s_qy[lr + 1][lc + 1], qyB = s_qy[lr + 2][lc + 1];
+      // components of this cell's own faces (left x-face, top y-face)
+      float cxL = s_qxc[lr + 1][lc + 1], fxL = s_qxf[lr + 1][lc + 1];
+      float cyT = s_qyc[lr + 1][lc + 1], fyT = s_qyf[lr + 1][lc + 1];
+      if (limit) {
+        const float bc = s_beta[lr + 1][lc + 1];
+        limit_components(qxL, cxL, fxL, s_beta[lr + 1][lc], bc);
+        limit_components(qyT, cyT, fyT, s_beta[lr][lc + 1], bc);
+        limit_faces(qxL, qxR, qyT, qyB, bc, s_beta[lr + 1][lc], s_beta[lr + 1][lc + 2],
+                    s_beta[lr][lc + 1], s_beta[lr + 2][lc + 1]);
+      }
+      const int64_t o = (int64_t)r * P + c;
+      if (flags & LI_SED) {  // own faces only: left (c >= 1) and top (r >= 1)
+        const ChanCell self = cell(lr + 2, lc + 2);
+        if (c >= 1) {
+          const float qs = combined_capacity(cxL, fxL, cell(lr + 2, lc + 1), self,
+                                             0.5f * (a.d50[o - 1] + a.d50[o]), dx, rdx);
+          float si = a.qs_int_x[o], sc = a.qs_comp_x[o];
+          kahan_add(si, sc, qs * k.dt);
+          a.qs_int_x[o] = si;
+          a.qs_comp_x[o] = sc;
+        }
+        if (r >= 1) {
+          const float qs = combined_capacity(cyT, fyT, cell(lr + 1, lc + 2), self,
+                                             0.5f * (a.d50[o - P] + a.d50[o]), dx, rdx);
+          float si = a.qs_int_y[o], sc = a.qs_comp_y[o];
+          kahan_add(si, sc, qs * k.dt);
+          a.qs_int_y[o] = si;
+          a.qs_comp_y[o] = sc;
+        }
+      }
+      float comp = (flags & LI_COMP) ? a.h_comp[o] : 0.0f;
+      const float rain_v = (flags & LI_FIELD) ? a.rain[o] : 0.0f;
+      const float infil_v = (flags & LI_INFIL) ? a.infil[o] : 0.0f;
+      const float h = cell_update(flags, r, c, ny, nx, s_h[lr + 1][lc + 1], qxL, qxR, qyT, qyB,
+                                  k, comp, rain_v, infil_v, sum);
+      m_col = fmx(m_col, column_depth_u(h, s_w[lr + 2][lc + 2], s_d[lr + 2][lc + 2], dx, rdx));
+      if (flags & LI_COMP) a.h_comp[o] = comp;
+      h_out[o] = h;
+      qx_out[o] = qxL;
+      qxc_out[o] = cxL;
+      qxf_out[o] = fxL;
+      if (c == nx - 1) {  // the closed east edge face: total and components 0
+        qx_out[o + 1] = qxR;
+        qxc_out[o + 1] = 0.0f;
+        qxf_out[o + 1] = 0.0f;
+      }
+      qy_out[o] = qyT;
+      qyc_out[o] = cyT;
+      qyf_out[o] = fyT;
+      if (r == ny - 1) {
+        qy_out[o + P] = qyB;
+        qyc_out[o + P] = 0.0f;
+        qyf_out[o + P] = 0.0f;
+      }
+    }
+  }
+  sum.m_h = m_col;
+  write_partials(a, tile, sum, s_red);
+}
+
 // One state: one block per tile.
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
@@ -592,6 +812,37 @@ static const LiVariant kLiVariants[] = {
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
 
+template <int BX, int BY>
+__global__ __launch_bounds__(kThreads) void li_chan_kernel(LiArgs a) {
+  li_chan_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
+}
+struct ChanVariant {
+  int bx, by;
+  StepFn kern;
+};
+static const ChanVariant kChanVariants[] = {
+    {64, 8, li_chan_kernel<64, 8>},
+    {32, 16, li_chan_kernel<32, 16>},
+    {32, 8, li_chan_kernel<32, 8>},
+    {16, 16, li_chan_kernel<16, 16>},
+};
+constexpr int kNumChanVariants = sizeof(kChanVariants) / sizeof(kChanVariants[0]);
+constexpr int kDefaultChanVariant = 0;
+static const ChanVariant &chan_variant_at(int v) {
+  if (v < 0 || v >= kNumChanVariants) v = kDefaultChanVariant;
+  return kChanVariants[v];
+}
+int chan_variant_count() { return kNumChanVariants; }
+int chan_default_variant() { return kDefaultChanVariant; }
+void chan_variant_shape(int v, int *bx, int *by) {
+  *bx = chan_variant_at(v).bx;
+  *by = chan_variant_at(v).by;
+}
+int chan_num_blocks(const rbs_state *s) {
+  const ChanVariant &v = chan_variant_at(s->chan_variant);
+  return ((s->nx + v.bx - 1) / v.bx) * ((s->ny + v.by - 1) / v.by);
+}
+
 static const LiVariant &variant_at(int v) {
   if (v < 0 || v >= kNumLiVariants) v = kDefaultLiVariant;
   return kLiVariants[v];
@@ -621,6 +872,7 @@ int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *til
 }
 
 int li_num_blocks(const rbs_state *s) {
+  if (s->armed & RBS_ARM_CHAN) return chan_num_blocks(s);
   const LiVariant &v = variant_at(s->li_variant);
   const int tx = (s->nx + v.bx - 1) / v.bx, ty = (s->ny + v.by - 1) / v.by;
   return tx * ty;
@@ -634,8 +886,9 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
     set_error("partial buffer too small (%d < %d)", s->part_cap, a.n_tiles);
     return RBS_E_STATE;
   }
-  hipLaunchKernelGGL((a.flags & LI_SED) ? v.kern_sed : v.kern, dim3(a.n_tiles), dim3(kThreads),
-                     0, s->stream, a);
+  StepFn fn = (a.flags & LI_SED) ? v.kern_sed : v.kern;
+  if (a.flags & LI_CHAN) fn = chan_variant_at(s->chan_variant).kern;
+  hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   s->cur ^= 1;
   return RBS_OK;
@@ -659,6 +912,15 @@ void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
   a.qs_comp_x = s->qs_comp_x;
   a.qs_int_y = s->qs_int_y;
   a.qs_comp_y = s->qs_comp_y;
+  a.ch_w = s->ch_w;
+  a.ch_d = s->ch_d;
+  a.ch_n = s->ch_n;
+  for (int k = 0; k < 2; ++k) {
+    a.qx_ch[k] = s->qx_ch[k];
+    a.qx_fp[k] = s->qx_fp[k];
+    a.qy_ch[k] = s->qy_ch[k];
+    a.qy_fp[k] = s->qy_fp[k];
+  }
   a.ctl = s->ctl;
   a.part = s->part;
   a.dx64 = s->dx;
@@ -666,6 +928,10 @@ void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
   a.ny = s->ny;
   a.nx = s->nx;
   a.tiles_x = (s->nx + v.bx - 1) / v.bx;
+  if (s->armed & RBS_ARM_CHAN) {
+    const ChanVariant &cv = chan_variant_at(s->chan_variant);
+    a.tiles_x = (s->nx + cv.bx - 1) / cv.bx;
+  }
   a.n_tiles = li_num_blocks(s);
   int f = 0;
   if (limit) f |= LI_LIMIT;
@@ -673,6 +939,7 @@ void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
   if (s->armed & RBS_ARM_RAIN) f |= LI_FIELD;
   if (s->armed & RBS_ARM_INFIL) f |= LI_INFIL;
   if (s->armed & RBS_ARM_SED) f |= LI_SED;
+  if (s->armed & RBS_ARM_CHAN) f |= LI_CHAN;
   if (s->bc[RBS_EDGE_WEST] == RBS_BC_OPEN) f |= LI_OPEN_W;
   if (s->bc[RBS_EDGE_EAST] == RBS_BC_OPEN) f |= LI_OPEN_E;
   if (s->bc[RBS_EDGE_NORTH] == RBS_BC_OPEN) f |= LI_OPEN_N;

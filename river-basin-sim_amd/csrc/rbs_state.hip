@@ -112,6 +112,13 @@ static int field_view(rbs_state *s, int f, FieldView *v) {
     case RBS_F_DZ_UNAPPLIED: cell(s->dz_unapplied, 8); break;
     case RBS_F_DZ_LO: cell(s->dz_lo); break;
     case RBS_F_DZ_HI: cell(s->dz_hi); break;
+    case RBS_F_CH_W: cell(s->ch_w); break;
+    case RBS_F_CH_D: cell(s->ch_d); break;
+    case RBS_F_CH_N: cell(s->ch_n); break;
+    case RBS_F_QX_CH: *v = {s->qx_ch[cur], ny, nx + 1, 4}; break;
+    case RBS_F_QX_FP: *v = {s->qx_fp[cur], ny, nx + 1, 4}; break;
+    case RBS_F_QY_CH: *v = {s->qy_ch[cur], ny + 1, nx, 4}; break;
+    case RBS_F_QY_FP: *v = {s->qy_fp[cur], ny + 1, nx, 4}; break;
     default:
       set_error("unknown field id %d", f);
       return RBS_E_ARG;
@@ -187,6 +194,21 @@ static int arm_fields(rbs_state *s, uint32_t arm) {
     if (rc) return rc;
     rc = fill_f32(s, s->dz_hi, ny, INFINITY);
     if (rc) return rc;
+  }
+  if ((arm & RBS_ARM_CHAN) && !s->ch_w) {
+    ALLOC(s->ch_w, ny, 4);
+    ALLOC(s->ch_d, ny, 4);
+    ALLOC(s->ch_n, ny, 4);
+    for (int k = 0; k < 2; ++k) {
+      ALLOC(s->qx_ch[k], ny, 4);
+      ALLOC(s->qx_fp[k], ny, 4);
+      ALLOC(s->qy_ch[k], ny + 1, 4);
+      ALLOC(s->qy_fp[k], ny + 1, 4);
+    }
+    // the channel roughness defaults to the floodplain's (channels.py:300-307)
+    RBS_HIP(hipMemcpyAsync(s->ch_n, s->n, (size_t)ny * s->pitch * 4, hipMemcpyDeviceToDevice,
+                           s->stream));
+    s->speed_scheme = -1;  // the CFL reduction now runs over the water column
   }
   s->armed |= arm;
   invalidate_graph(s);
@@ -281,6 +303,11 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
     nblk = std::max(nblk, hllc_num_blocks(s));
   }
   s->hllc_variant = hllc_default_variant();
+  for (int v = 0; v < chan_variant_count(); ++v) {
+    s->chan_variant = v;
+    nblk = std::max(nblk, chan_num_blocks(s));
+  }
+  s->chan_variant = chan_default_variant();
   TRY(ensure_partials(s, nblk));
   if (arm) TRY(arm_fields(s, arm));
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "sync"));
@@ -304,7 +331,9 @@ static int state_destroy(rbs_state *s) {
                   s->dz_lo, s->dz_hi, s->dz_cum, s->dz_unapplied, s->sed_part, s->part.speed_max,
                   s->part.vol, s->part.infil, s->part.edge, s->ctl, s->d_sync,
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
-                  s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol};
+                  s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol, s->ch_w, s->ch_d,
+                  s->ch_n, s->qx_ch[0], s->qx_ch[1], s->qx_fp[0], s->qx_fp[1], s->qy_ch[0],
+                  s->qy_ch[1], s->qy_fp[0], s->qy_fp[1]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &c : s->stage) {
@@ -322,7 +351,7 @@ static int state_destroy(rbs_state *s) {
 int rbs_arm(rbs_state *s, uint32_t arm) {
   RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
-  if (arm & ~0x1Fu) return set_error("unknown arm flags 0x%x", arm), RBS_E_ARG;
+  if (arm & ~0x3Fu) return set_error("unknown arm flags 0x%x", arm), RBS_E_ARG;
   return arm_fields(s, arm);
 }
 
@@ -375,7 +404,9 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
   }
   rc = copy_rows(s, v, const_cast<void *>(host), true);
   if (rc) return rc;
-  if (field == RBS_F_H || field == RBS_F_HU || field == RBS_F_HV) s->speed_scheme = -1;
+  if (field == RBS_F_H || field == RBS_F_HU || field == RBS_F_HV || field == RBS_F_CH_W ||
+      field == RBS_F_CH_D)
+    s->speed_scheme = -1;
   if (field == RBS_F_RAIN) {
     // run.py:276: the ledger's field-rain inflow uses sum f64(field)
     const float *f = (const float *)host;
@@ -391,7 +422,8 @@ static int download_field(rbs_state *s, int32_t field, void *host, size_t bytes)
 int rbs_download(rbs_state *s, int32_t field, void *host, size_t bytes) {
   if (!s || !host) return set_error("null argument"), RBS_E_ARG;
   if (field == RBS_F_ETA) {
-    // eta = h + z (local_inertial.py:61-69), formed on the host from the two fields.
+    // eta = h + z (local_inertial.py:61-69), or the storage curve of a channelled
+    // state (channels.py:59-72), formed on the host in float32 from the fields.
     const size_t want = (size_t)s->ny * s->nx * 4;
     if (bytes != want) return set_error("eta expects %zu bytes", want), RBS_E_ARG;
     std::vector<float> z((size_t)s->ny * s->nx);
@@ -400,7 +432,17 @@ int rbs_download(rbs_state *s, int32_t field, void *host, size_t bytes) {
     rc = download_field(s, RBS_F_Z, z.data(), bytes);
     if (rc) return rc;
     float *e = (float *)host;
-    for (size_t k = 0; k < z.size(); ++k) e[k] = e[k] + z[k];
+    if (!s->ch_w) {
+      for (size_t k = 0; k < z.size(); ++k) e[k] = e[k] + z[k];
+      return RBS_OK;
+    }
+    std::vector<float> w(z.size()), d(z.size());
+    rc = download_field(s, RBS_F_CH_W, w.data(), bytes);
+    if (rc) return rc;
+    rc = download_field(s, RBS_F_CH_D, d.data(), bytes);
+    if (rc) return rc;
+    const float dx = (float)s->dx;
+    for (size_t k = 0; k < z.size(); ++k) e[k] = eta_subgrid(e[k], z[k], w[k], d[k], dx);
     return RBS_OK;
   }
   return download_field(s, field, host, bytes);
@@ -697,6 +739,34 @@ int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is
 int rbs_hllc_variant_count(int32_t *count) {
   if (!count) return set_error("null count"), RBS_E_ARG;
   *count = hllc_variant_count();
+  return RBS_OK;
+}
+
+int rbs_set_chan_variant(rbs_state *s, int32_t variant) {
+  RBS_API_SHARED;
+  if (!s) return set_error("null state"), RBS_E_ARG;
+  if (variant < 0 || variant >= chan_variant_count())
+    return set_error("channel variant %d is not in [0, %d)", variant, chan_variant_count()), RBS_E_ARG;
+  s->chan_variant = variant;
+  invalidate_graph(s);
+  return RBS_OK;
+}
+
+int rbs_chan_variant_count(int32_t *count) {
+  if (!count) return set_error("null count"), RBS_E_ARG;
+  *count = chan_variant_count();
+  return RBS_OK;
+}
+
+int rbs_chan_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default) {
+  if (!bx || !by || !is_default) return set_error("null argument"), RBS_E_ARG;
+  if (variant < 0 || variant >= chan_variant_count())
+    return set_error("channel variant %d is not in [0, %d)", variant, chan_variant_count()), RBS_E_ARG;
+  int x, y;
+  chan_variant_shape(variant, &x, &y);
+  *bx = x;
+  *by = y;
+  *is_default = variant == chan_default_variant() ? 1 : 0;
   return RBS_OK;
 }
 

@@ -19,8 +19,9 @@ OK, E_ARG, E_STATE, E_HIP, E_NOMEM, E_RANGE = 0, -1, -2, -3, -4, -5
 F_H, F_Z, F_N, F_QX, F_QY, F_HU, F_HV, F_RAIN, F_INFIL, F_H_COMP = range(10)
 F_D50, F_QS_INT_X, F_QS_COMP_X, F_QS_INT_Y, F_QS_COMP_Y, F_Z0 = range(10, 16)
 F_DZ_CUM, F_DZ_UNAPPLIED, F_DZ_LO, F_DZ_HI, F_ETA = range(16, 21)
+F_CH_W, F_CH_D, F_CH_N, F_QX_CH, F_QX_FP, F_QY_CH, F_QY_FP = range(21, 28)
 
-ARM_COMP, ARM_RAIN, ARM_INFIL, ARM_HLLC, ARM_SED = 1, 2, 4, 8, 16
+ARM_COMP, ARM_RAIN, ARM_INFIL, ARM_HLLC, ARM_SED, ARM_CHAN = 1, 2, 4, 8, 16, 32
 EDGE = {"west": 0, "east": 1, "north": 2, "south": 3}
 BC = {"closed": 0, "open": 1, "fixed_stage": 2}
 SCHEME_LI, SCHEME_HLLC = 0, 1
@@ -105,6 +106,9 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_li_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32, _pi32]),
     "rbs_batch_set_variant": (ctypes.c_int, [_vp, _i32, _i32]),
     "rbs_batch_get_variant": (ctypes.c_int, [_vp, _pi32, _pi32]),
+    "rbs_set_chan_variant": (ctypes.c_int, [_vp, _i32]),
+    "rbs_chan_variant_count": (ctypes.c_int, [_pi32]),
+    "rbs_chan_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),
     "rbs_set_hllc_variant": (ctypes.c_int, [_vp, _i32]),
     "rbs_hllc_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_hllc_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),

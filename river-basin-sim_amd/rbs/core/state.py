@@ -66,7 +66,7 @@ class State:
         self.boundaries: dict[str, str] = {e: "closed" for e in _EDGES}
         self.open_edges: frozenset[str] = frozenset()
         self.stage_curves: dict[str, list[tuple[float, float]]] = {}
-        self.channels = None  # sub-grid channels are not part of this build
+        self.channels = None  # rbs.core.channels.ChannelState once armed
         self.sediment = None  # rbs.core.sediment.SedimentState once armed
         self.hllc = None
 

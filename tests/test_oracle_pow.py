@@ -1,8 +1,8 @@
-"""The oracle's powers (x^(7/3), x^(1/3), x^(3/2)) are the correctly rounded float32
-values of the exact powers.
+"""The oracle's powers (x^(7/3), x^(4/3), x^(1/3), x^(3/2)) are the correctly rounded
+float32 values of the exact powers.
 
 The reference takes these powers with Warp's wp.pow, a libm powf good to about an
-ulp (friction.py:117, sediment.py:168, :192).  The oracle and the HIP kernels both
+ulp (friction.py:57, :117, sediment.py:168, :192).  The oracle and the HIP kernels both
 evaluate them by the same sequence of IEEE basic operations (so the GPU and the
 oracle agree bit for bit); this pins that sequence to the mathematics: against the
 exact power computed to 100 decimal digits, rounded once to float32.
@@ -17,8 +17,8 @@ import pytest
 mpmath.mp.dps = 40
 
 
-@pytest.mark.parametrize("name,expo", [("pow73", mpmath.mpf(7) / 3), ("cbrt", mpmath.mpf(1) / 3),
-                                       ("pow15", mpmath.mpf(3) / 2)])
+@pytest.mark.parametrize("name,expo", [("pow73", mpmath.mpf(7) / 3), ("pow43", mpmath.mpf(4) / 3),
+                                       ("cbrt", mpmath.mpf(1) / 3), ("pow15", mpmath.mpf(3) / 2)])
 def test_powers_are_correctly_rounded(orc, name, expo):
     fn = getattr(orc, name)
     rng = np.random.default_rng(hash(name) % 2**32)
