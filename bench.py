@@ -110,7 +110,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="li",
-                    choices=["li", "ensemble", "serve", "hllc", "copy"])
+                    choices=["li", "ensemble", "serve", "hllc", "copy", "channels"])
     ap.add_argument("--sweep", action="store_true",
                     help="also time every LI kernel variant on the spun-up state")
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
@@ -142,6 +142,8 @@ def main() -> int:
         return bench_hllc(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "channels":
+        return bench_channels(args)
     st, d = build_state(args.n, args.seed)
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
@@ -491,6 +493,112 @@ def serve_capacity(reqs, chunk: int, variants=None) -> dict:
             "plain_launches_scenarios_per_h": round(3600.0 * n / t_plain, 1),
             "speedup_vs_plain_launches": round(t_plain / st["wall_s"], 3),
             "bitwise_identical_to_plain": bool(same)}
+
+
+CHAN_BYTES_PER_CELL = 84  # + w, d, n_ch, the four old components, and the four new
+
+
+def bench_channels(args) -> int:
+    """Reach-scale basin with sub-grid channels: 4096 x 4096 cells of 100 m, rivers
+    sized from drained area, rain field (Kahan) + infiltration, open edges; the fused
+    channel kernel device-driven from a captured graph."""
+    from rbs import _lib, terrain
+    from rbs.core.channels import arm_channels
+    from rbs.core.state import State
+    from rbs.driver import DeviceRun
+    from rbs.scheduler import MultiRateScheduler
+
+    n = args.n
+    t0 = time.time()
+    d = terrain.reach_basin(n, dx=100.0, seed=args.seed)
+    log(f"[channels] reach basin {n}x{n} generated in {time.time() - t0:.0f} s, "
+        f"{100 * float((d['width'] > 0).mean()):.1f}% channel cells")
+    st = State.from_bed(d["bed"], d["dx"], depth=0.0, manning=d["manning"])
+    st.set_rain_field(d["rain"])
+    st.set_infiltration(d["infil"])
+    st.set_open_boundaries({e: "open" for e in ("north", "south", "east", "west")})
+    st.arm_source_compensation()
+    arm_channels(st, d["width"], d["depth"], d["chan_manning"])
+    if args.variant >= 0:
+        _lib.call("rbs_set_chan_variant", st._handle, args.variant)
+    cells = n * n
+    rain_sum = float(d["rain"].astype(np.float64).sum())
+    end_time = 21600.0
+    sched = MultiRateScheduler(end_time=end_time, output_every=1800.0, events=[7200.0])
+    run = DeviceRun(st, "local_inertial", sched, alpha=0.7, dt_max=30.0, rain_is_field=True,
+                    rain_duration=7200.0, rain_field_sum=rain_sum)
+    t0 = time.time()
+    n_spin = 0
+    for seg in sched.segments():
+        if seg.t_stop > args.spinup + 1e-6:
+            break
+        n_spin += run.run_until(seg.t_stop)
+    log(f"[channels] spin-up to t={st.ledger().t:.0f} s: {n_spin} steps in {time.time() - t0:.1f} s")
+    run.run_until(end_time, max_steps=args.warmup)
+    st.sync()
+    led0 = st.ledger()
+    t0 = time.perf_counter()
+    taken = run.run_until(end_time, max_steps=args.steps)
+    st.sync()
+    el = time.perf_counter() - t0
+    led1 = st.ledger()
+    value = cells * taken / el / 1e9
+    log(f"[channels] timed {taken} steps in {el:.3f} s: {value:.2f} Gcell/s; "
+        f"t {led0.t:.0f} -> {led1.t:.0f} s, deepest column {led1.speed_max:.2f} m")
+    dt_k = float(led1.dt) if led1.dt > 0 else 1.0
+    avg = ctypes.c_double()
+    _lib.call("rbs_time_li_kernel", st._handle, args.kernel_reps, dt_k, 0.0, 1.0,
+              ctypes.byref(avg))
+    kern_us = avg.value * 1e3
+    achieved = CHAN_BYTES_PER_CELL * cells / (kern_us * 1e-6) / 1e9
+    log(f"[channels] fused channel kernel {kern_us:.1f} us -> {achieved:.0f} GB/s "
+        f"({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    sweep = None
+    if args.sweep:
+        count = ctypes.c_int32()
+        _lib.call("rbs_chan_variant_count", ctypes.byref(count))
+        sweep = []
+        for v in range(count.value):
+            bx, by, dflt = (ctypes.c_int32() for _ in range(3))
+            _lib.call("rbs_chan_variant_info", v, ctypes.byref(bx), ctypes.byref(by),
+                      ctypes.byref(dflt))
+            _lib.call("rbs_set_chan_variant", st._handle, v)
+            _lib.call("rbs_time_li_kernel", st._handle, 3, dt_k, 0.0, 1.0, ctypes.byref(avg))
+            _lib.call("rbs_time_li_kernel", st._handle, args.kernel_reps, dt_k, 0.0, 1.0,
+                      ctypes.byref(avg))
+            us = avg.value * 1e3
+            gbs = CHAN_BYTES_PER_CELL * cells / (us * 1e-6) / 1e9
+            sweep.append({"variant": v, "tile": f"{bx.value}x{by.value}",
+                          "default": bool(dflt.value), "kernel_us": round(us, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            log(f"[channels] variant {v} {bx.value}x{by.value}: {us:.1f} us, "
+                f"{100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": taken,
+        "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3 / max(taken, 1), 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"li_subgrid_channels_{n}", "grid": f"{n}x{n}", "dx_m": 100.0,
+                   "scheme": "local_inertial + sub-grid channels",
+                   "channel_cells": int((d["width"] > 0).sum()),
+                   "sources": "lognormal rain field (Kahan) + infiltration, open edges"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel_us": round(kern_us, 2),
+                     "bytes_per_cell": CHAN_BYTES_PER_CELL},
+    }
+    if sweep:
+        out["variant_sweep"] = sweep
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def valley(n: int, dx: float, seed: int):

@@ -812,22 +812,217 @@ static const LiVariant kLiVariants[] = {
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
 
+// ---- family 4: register strips of a state with sub-grid channels ----------------------
+// li_strip's rolling window with the channel forms: each row's cells carry the
+// storage-curve surface and the section (z, n, w, d, n_ch); a face's two components
+// travel beside its total through the window, and the finished row limits its own
+// faces' components by the same donor factor as their totals.
+template <int H, bool SED>
+__device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
+  const DevCtl *ctl = a.ctl;
+  if (ctl->active != 1) return;
+  const int cur = ctl->cur;
+  const float *__restrict__ h_in = a.h[cur];
+  float *__restrict__ h_out = a.h[cur ^ 1];
+  const float *__restrict__ qxc_in = a.qx_ch[cur];
+  const float *__restrict__ qxf_in = a.qx_fp[cur];
+  const float *__restrict__ qyc_in = a.qy_ch[cur];
+  const float *__restrict__ qyf_in = a.qy_fp[cur];
+  float *__restrict__ qx_out = a.qx[cur ^ 1];
+  float *__restrict__ qy_out = a.qy[cur ^ 1];
+  float *__restrict__ qxc_out = a.qx_ch[cur ^ 1];
+  float *__restrict__ qxf_out = a.qx_fp[cur ^ 1];
+  float *__restrict__ qyc_out = a.qy_ch[cur ^ 1];
+  float *__restrict__ qyf_out = a.qy_fp[cur ^ 1];
+  __shared__ double s_red[4 * (kThreads / 64)];
+
+  const int ny = a.ny, nx = a.nx;
+  const int64_t P = a.pitch;
+  const int flags = a.flags;
+  const bool limit = flags & LI_LIMIT;
+  const StepScalars k = step_scalars(a, ctl);
+  const float dx = (float)a.dx64;
+  const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+  const int c = tx * kStripW - 2 + lane;
+  const int y0 = (ty * kStripWaves + wave) * H;
+  const int y_end = y0 + H < ny ? y0 + H : ny;
+  const uint32_t cc = clampi(c, 0, nx - 1);
+  const uint32_t cf = clampi(c, 0, nx);
+  const uint32_t cu = (uint32_t)c;  // used only where 0 <= c < nx
+  const bool col_ok = c >= 0 && c < nx;
+  const bool xface_ok = c >= 1 && c <= nx - 1;
+  const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
+
+  CellSums sum;
+  float m_col = 0.0f;  // reduce_column_max starts at 0
+  if (y0 < ny) {  // wave-uniform
+    // window: the cells of rows R-1 (c1), R-2 and R-3 (c2, c3: bedload only; else
+    // just c2's width and depth for the column depth), the totals and components of
+    // the faces of rows R-1 and R-2, old depths of rows R-1 and R-2, donor factors of
+    // rows R-2 and R-3
+    ChanCell c1{}, c2{}, c3{};
+    float fx1 = 0.0f, fx2 = 0.0f, fy1 = 0.0f, fy2 = 0.0f;
+    float fxc1 = 0.0f, fxf1 = 0.0f, fxc2 = 0.0f, fxf2 = 0.0f;
+    float fyc1 = 0.0f, fyf1 = 0.0f, fyc2 = 0.0f, fyf2 = 0.0f;
+    float h1 = 0.0f, h2 = 0.0f, b2 = 1.0f, b3 = 1.0f;
+    // prefetched row: h, z, n, w, d, n_ch, old x-face and y-face components
+    float ph, pz, pn, pw, pd, pnc, pxc, pxf, pyc, pyf;
+    float pcomp = 0.0f, prain = 0.0f, pinfil = 0.0f;
+    auto ingest = [&](int R) {
+      const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;  // uniform
+      const int64_t oy = (int64_t)clampi(R, 0, ny) * P;
+      ph = at(h_in + o, cc);
+      pz = at(a.z + o, cc);
+      pn = at(a.n + o, cc);
+      pw = at(a.ch_w + o, cc);
+      pd = at(a.ch_d + o, cc);
+      pnc = at(a.ch_n + o, cc);
+      pxc = at(qxc_in + o, cf);
+      pxf = at(qxf_in + o, cf);
+      pyc = at(qyc_in + oy, cc);
+      pyf = at(qyf_in + oy, cc);
+    };
+    auto sources = [&](int i) {
+      if (i >= y0 && i < y_end && own) {
+        const int64_t o = (int64_t)i * P;
+        if (flags & LI_COMP) pcomp = at(a.h_comp + o, cu);
+        if (flags & LI_FIELD) prain = at(a.rain + o, cu);
+        if (flags & LI_INFIL) pinfil = at(a.infil + o, cu);
+      }
+    };
+    ingest(y0 - 2);
+    c1 = ChanCell{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};
+    h1 = ph;
+    ingest(y0 - 1);
+    for (int it = 1; it < H + 4; ++it) {
+      const int R = y0 - 2 + it;
+      const ChanCell c0{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};
+      const float h0 = ph;
+      float qxc = pxc, qxf = pxf, qyc = pyc, qyf = pyf;
+      float comp = pcomp;
+      const float rain_v = prain, infil_v = pinfil;
+      if (it + 1 < H + 4) ingest(R + 1);
+      sources(R - 1);
+
+      // x-faces of row R and y-faces above row R, both components
+      const ChanCell cl{from_left(c0.eta), from_left(c0.z), from_left(c0.n),
+                        from_left(c0.w), from_left(c0.d), from_left(c0.nc)};
+      const bool fx_valid = xface_ok && R >= 0 && R < ny;
+      const bool fy_valid = col_ok && R >= 1 && R <= ny - 1;
+      float fx0 = 0.0f, fy0 = 0.0f;
+      if (fx_valid) fx0 = chan_face(cl, c0, qxc, qxf, dx, rdx, k.dt);
+      else qxc = qxf = 0.0f;
+      if (fy_valid) fy0 = chan_face(c1, c0, qyc, qyf, dx, rdx, k.dt);
+      else qyc = qyf = 0.0f;
+
+      // donor factor of row R-1 from the totals
+      const float fx1r = from_right(fx1);
+      const float b1 = limit ? donor_beta(fx1, fx1r, fy1, fy0, h1, k.dtdx) : 1.0f;
+
+      // finish row i = R-2
+      const int i = R - 2;
+      const float fx2r = from_right(fx2);
+      const float b2l = from_left(b2), b2r = from_right(b2);
+      ChanCell c2l{};
+      if (SED) c2l = ChanCell{from_left(c2.eta), from_left(c2.z), from_left(c2.n),
+                              from_left(c2.w), from_left(c2.d), from_left(c2.nc)};
+      if (i >= y0 && i < y_end && own) {
+        float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
+        float cxL = fxc2, fxL = fxf2, cyT = fyc2, fyT = fyf2;
+        if (limit) {
+          limit_components(qxL, cxL, fxL, b2l, b2);
+          limit_components(qyT, cyT, fyT, b3, b2);
+          limit_faces(qxL, qxR, qyT, qyB, b2, b2l, b2r, b3, b1);
+        }
+        const int64_t row = (int64_t)i * P;  // uniform
+        const int64_t o = row + c;
+        if (SED && (flags & LI_SED)) {
+          if (c >= 1) {
+            const float qs = combined_capacity(cxL, fxL, c2l, c2,
+                                               0.5f * (a.d50[o - 1] + a.d50[o]), dx, rdx);
+            float si = a.qs_int_x[o], sc = a.qs_comp_x[o];
+            kahan_add(si, sc, qs * k.dt);
+            a.qs_int_x[o] = si;
+            a.qs_comp_x[o] = sc;
+          }
+          if (i >= 1) {
+            const float qs = combined_capacity(cyT, fyT, c3, c2,
+                                               0.5f * (a.d50[o - P] + a.d50[o]), dx, rdx);
+            float si = a.qs_int_y[o], sc = a.qs_comp_y[o];
+            kahan_add(si, sc, qs * k.dt);
+            a.qs_int_y[o] = si;
+            a.qs_comp_y[o] = sc;
+          }
+        }
+        const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
+                                    rain_v, infil_v, sum);
+        m_col = fmx(m_col, column_depth_u(h, c2.w, c2.d, dx, rdx));
+        if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
+        at(h_out + row, cu) = h;
+        at(qx_out + row, cu) = qxL;
+        at(qxc_out + row, cu) = cxL;
+        at(qxf_out + row, cu) = fxL;
+        if (c == nx - 1) {  // the closed east edge face: total and components 0
+          at(qx_out + row, cu + 1) = qxR;
+          at(qxc_out + row, cu + 1) = 0.0f;
+          at(qxf_out + row, cu + 1) = 0.0f;
+        }
+        at(qy_out + row, cu) = qyT;
+        at(qyc_out + row, cu) = cyT;
+        at(qyf_out + row, cu) = fyT;
+        if (i == ny - 1) {
+          at(qy_out + row + P, cu) = qyB;
+          at(qyc_out + row + P, cu) = 0.0f;
+          at(qyf_out + row + P, cu) = 0.0f;
+        }
+      }
+      // slide the window
+      if (SED) c3 = c2;
+      c2 = c1;
+      c1 = c0;
+      fx2 = fx1; fx1 = fx0;
+      fy2 = fy1; fy1 = fy0;
+      fxc2 = fxc1; fxc1 = qxc;
+      fxf2 = fxf1; fxf1 = qxf;
+      fyc2 = fyc1; fyc1 = qyc;
+      fyf2 = fyf1; fyf1 = qyf;
+      h2 = h1; h1 = h0;
+      b3 = b2; b2 = b1;
+    }
+  }
+  sum.m_h = m_col;
+  write_partials(a, tile, sum, s_red);
+}
+
 template <int BX, int BY>
 __global__ __launch_bounds__(kThreads) void li_chan_kernel(LiArgs a) {
   li_chan_tile<BX, BY>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
+// W: the waves per SIMD the register allocator aims for; the bedload form keeps its
+// two extra rows of cells at 3 (fewer registers would spill)
+template <int H, bool SED, int W>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 3 : W)))
+void li_chan_strip_kernel(LiArgs a) {
+  li_chan_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
+}
 struct ChanVariant {
   int bx, by;
-  StepFn kern;
+  StepFn kern, kern_sed;
 };
+#define RBS_CHAN_TILE(X, Y) {X, Y, li_chan_kernel<X, Y>, li_chan_kernel<X, Y>}
+#define RBS_CHAN_STRIP(H, W)                                                    \
+  {kStripW, kStripWaves * (H), li_chan_strip_kernel<H, false, W>,               \
+   li_chan_strip_kernel<H, true, W>}
 static const ChanVariant kChanVariants[] = {
-    {64, 8, li_chan_kernel<64, 8>},
-    {32, 16, li_chan_kernel<32, 16>},
-    {32, 8, li_chan_kernel<32, 8>},
-    {16, 16, li_chan_kernel<16, 16>},
+    RBS_CHAN_TILE(64, 8), RBS_CHAN_TILE(32, 16), RBS_CHAN_TILE(32, 8), RBS_CHAN_TILE(16, 16),
+    RBS_CHAN_STRIP(8, 4), RBS_CHAN_STRIP(16, 4), RBS_CHAN_STRIP(32, 4), RBS_CHAN_STRIP(4, 4),
 };
 constexpr int kNumChanVariants = sizeof(kChanVariants) / sizeof(kChanVariants[0]);
-constexpr int kDefaultChanVariant = 0;
+constexpr int kDefaultChanVariant = 4;  // strips of 8 rows: fastest measured at 4096^2
 static const ChanVariant &chan_variant_at(int v) {
   if (v < 0 || v >= kNumChanVariants) v = kDefaultChanVariant;
   return kChanVariants[v];
@@ -887,7 +1082,10 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
     return RBS_E_STATE;
   }
   StepFn fn = (a.flags & LI_SED) ? v.kern_sed : v.kern;
-  if (a.flags & LI_CHAN) fn = chan_variant_at(s->chan_variant).kern;
+  if (a.flags & LI_CHAN) {
+    const ChanVariant &cv = chan_variant_at(s->chan_variant);
+    fn = (a.flags & LI_SED) ? cv.kern_sed : cv.kern;
+  }
   hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   s->cur ^= 1;

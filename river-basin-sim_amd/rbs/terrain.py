@@ -11,7 +11,10 @@ the scenarios the benchmark names, deterministically from a seed:
 * ``voronoi_field`` -- piecewise-constant patches (e.g. Manning n);
 * ``lognormal_field`` -- ``median * exp(sigma * G)`` with G a unit-variance
   Gaussian field smoothed to a correlation length;
-* ``uniform_field`` -- i.i.d. uniform cells.
+* ``uniform_field`` -- i.i.d. uniform cells;
+* ``reach_basin`` -- a coarse reach-scale basin whose rivers are carried as
+  sub-grid channels sized from drained area by the reference pipeline's hydraulic
+  geometry law (pipeline/channels.py:98-103, :767-820).
 """
 
 from __future__ import annotations
@@ -54,6 +57,21 @@ def fbm_dem(ny: int, nx: int, *, hurst: float = 0.8, relief: float = 200.0,
     surf -= surf.min()
     surf *= relief / max(surf.max(), 1e-12)
     return surf.astype(np.float32)
+
+
+def d8_area(dem: np.ndarray, dx: float) -> tuple[np.ndarray, np.ndarray]:
+    """Pit-filled surface and D8 drained area (cells, including the cell itself)."""
+    L = _terrain_lib()
+    ny, nx = dem.shape
+    z = np.ascontiguousarray(dem, dtype=np.float32).copy()
+    fp = ctypes.POINTER(ctypes.c_float)
+    if L.terrain_fill(ny, nx, z.ctypes.data_as(fp), ctypes.c_float(1e-4)) != 0:
+        raise MemoryError("terrain_fill")
+    acc = np.empty_like(z)
+    if L.terrain_d8_accumulate(ny, nx, z.ctypes.data_as(fp), ctypes.c_float(dx),
+                               acc.ctypes.data_as(fp)) != 0:
+        raise MemoryError("terrain_d8_accumulate")
+    return z, acc
 
 
 def carve_channels(dem: np.ndarray, dx: float, *, area_threshold_m2: float = 1.0e6,
@@ -137,3 +155,39 @@ def rain_on_terrain(n: int, *, dx: float = 10.0, seed: int = 20261203) -> dict:
     infil = uniform_field(n, n, 0.0, 10.0, seed=seed + 3) * np.float32(MM_H)
     return {"bed": bed, "manning": manning, "rain": rain.astype(np.float32),
             "infil": infil.astype(np.float32), "dx": dx}
+
+
+def hydraulic_geometry(area_km2: np.ndarray, dx: float, *, width_coef: float = 8.0,
+                       width_exp: float = 0.5, depth_coef: float = 0.27,
+                       depth_exp: float = 0.3, min_area_km2: float = 1.0):
+    """Channel width and bank-full depth from drained area, w = 8 A^0.5 m and
+    d = 0.27 A^0.3 m for A >= 1 km^2 (the reference pipeline's defaults), widths
+    capped at the largest float32 not above dx; "no channel" is 0 in both."""
+    area = np.asarray(area_km2, dtype=np.float64)
+    river = area >= min_area_km2
+    width = np.where(river, width_coef * np.power(np.maximum(area, 0.0), width_exp), 0.0)
+    depth = np.where(river, depth_coef * np.power(np.maximum(area, 0.0), depth_exp), 0.0)
+    cap = np.float32(dx)
+    if float(cap) > float(dx):
+        cap = np.nextafter(cap, np.float32(0.0))
+    width = np.minimum(width, float(cap))
+    depth = np.where(width > 0.0, depth, 0.0)
+    return width.astype(np.float32), depth.astype(np.float32)
+
+
+def reach_basin(n: int, *, dx: float = 100.0, seed: int = 20261203) -> dict:
+    """A coarse basin for the sub-grid channel path: n x n cells of dx (100 m), fBm
+    (H 0.8, 800 m relief), pit-filled; rivers wherever the D8 drained area reaches
+    1 km^2, carried as sub-grid channels (w = 8 A^0.5, d = 0.27 A^0.3, channel n
+    0.03); floodplain Manning in [0.04, 0.10]; lognormal rain (median 15 mm/h, sigma
+    0.6, 20 km correlation); infiltration U[0, 10] mm/h.  Rates in m/s."""
+    dem = fbm_dem(n, n, hurst=0.8, relief=800.0, seed=seed)
+    bed, acc = d8_area(dem, dx)
+    width, depth = hydraulic_geometry(acc.astype(np.float64) * dx * dx / 1.0e6, dx)
+    manning = voronoi_field(n, n, sites=512, lo=0.04, hi=0.10, seed=seed + 1)
+    rain = lognormal_field(n, n, median=15.0, sigma=0.6, corr_cells=20000.0 / dx,
+                           seed=seed + 2) * np.float32(MM_H)
+    infil = uniform_field(n, n, 0.0, 10.0, seed=seed + 3) * np.float32(MM_H)
+    return {"bed": bed, "manning": manning, "rain": rain.astype(np.float32),
+            "infil": infil.astype(np.float32), "dx": dx, "width": width, "depth": depth,
+            "chan_manning": np.full((n, n), 0.03, np.float32)}
