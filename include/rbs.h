@@ -195,6 +195,37 @@ int rbs_exner_activate(rbs_state *st, double porosity);
  * sum dz_unapplied}. */
 int rbs_sed_totals(rbs_state *st, double *totals);
 
+/* ---- output frames (run.py:389-400, state.py:232-256, zarr_writer.py:117-154) ------ */
+/* Fields of a frame: depth h, cell-centred u and v (0 where h < H_DRY; from the face
+ * discharges, or the momentum of an HLLC state), and the float64 bed change narrowed
+ * to float32.  u and v come together. */
+#define RBS_FRAME_DEPTH (1u << 0)
+#define RBS_FRAME_U (1u << 1)
+#define RBS_FRAME_V (1u << 2)
+#define RBS_FRAME_BED_CHANGE (1u << 3)
+typedef struct rbs_frames rbs_frames;
+/* A queue of `slots` frames of `st`: a device slot and a pinned host slot each, and a
+ * copy stream of its own.  Destroy it before its state.  Captures come from one host
+ * thread; waits and releases may come from another (a writer thread). */
+int rbs_frames_create(rbs_state *st, uint32_t fields, int32_t slots, rbs_frames **out);
+int rbs_frames_destroy(rbs_frames *q);
+/* Enqueue a frame of the state as it stands after the work already enqueued on the
+ * state's stream (e.g. when rbs_run_until returns): one kernel on that stream forms
+ * the fields and the clock into a free slot, then the queue's copy stream moves them
+ * to the host while the state's stream goes on.  RBS_E_STATE when every slot is held. */
+int rbs_frames_capture(rbs_frames *q, int32_t *slot);
+/* *ready = 1 once the slot's frame is on the host. */
+int rbs_frames_ready(rbs_frames *q, int32_t slot, int32_t *ready);
+/* Block until the slot's frame is on the host; the pointers (any may be NULL) receive
+ * the frame's clock and its dense (ny, nx) float32 planes in pinned memory (NULL for
+ * fields the queue does not carry), valid until the slot is released. */
+int rbs_frames_wait(rbs_frames *q, int32_t slot, double *t, float **depth, float **u, float **v,
+                    float **bed_change);
+/* Hand the slot back for reuse. */
+int rbs_frames_release(rbs_frames *q, int32_t slot);
+/* Device bytes held by the queue (pinned host bytes are the dense equivalent). */
+int rbs_frames_bytes(const rbs_frames *q, uint64_t *bytes);
+
 /* ---- ledger and device-driven runs -------------------------------------------------- */
 /* MassLedger inputs: device volume, sink totals, clock (massbalance.py:109-135). */
 int rbs_ledger_read(rbs_state *st, rbs_ledger *out);

@@ -22,6 +22,7 @@ F_DZ_CUM, F_DZ_UNAPPLIED, F_DZ_LO, F_DZ_HI, F_ETA = range(16, 21)
 F_CH_W, F_CH_D, F_CH_N, F_QX_CH, F_QX_FP, F_QY_CH, F_QY_FP = range(21, 28)
 
 ARM_COMP, ARM_RAIN, ARM_INFIL, ARM_HLLC, ARM_SED, ARM_CHAN = 1, 2, 4, 8, 16, 32
+FRAME_DEPTH, FRAME_U, FRAME_V, FRAME_BED_CHANGE = 1, 2, 4, 8
 EDGE = {"west": 0, "east": 1, "north": 2, "south": 3}
 BC = {"closed": 0, "open": 1, "fixed_stage": 2}
 SCHEME_LI, SCHEME_HLLC = 0, 1
@@ -106,6 +107,13 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_li_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32, _pi32]),
     "rbs_batch_set_variant": (ctypes.c_int, [_vp, _i32, _i32]),
     "rbs_batch_get_variant": (ctypes.c_int, [_vp, _pi32, _pi32]),
+    "rbs_frames_create": (ctypes.c_int, [_vp, ctypes.c_uint32, _i32, ctypes.POINTER(_vp)]),
+    "rbs_frames_destroy": (ctypes.c_int, [_vp]),
+    "rbs_frames_capture": (ctypes.c_int, [_vp, _pi32]),
+    "rbs_frames_ready": (ctypes.c_int, [_vp, _i32, _pi32]),
+    "rbs_frames_wait": (ctypes.c_int, [_vp, _i32, _pdbl] + [ctypes.POINTER(ctypes.POINTER(ctypes.c_float))] * 4),
+    "rbs_frames_release": (ctypes.c_int, [_vp, _i32]),
+    "rbs_frames_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "rbs_set_chan_variant": (ctypes.c_int, [_vp, _i32]),
     "rbs_chan_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_chan_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),
