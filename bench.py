@@ -110,7 +110,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="li",
-                    choices=["li", "ensemble", "serve", "hllc", "copy", "channels"])
+                    choices=["li", "ensemble", "serve", "hllc", "copy", "channels", "output"])
     ap.add_argument("--sweep", action="store_true",
                     help="also time every LI kernel variant on the spun-up state")
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
@@ -144,6 +144,8 @@ def main() -> int:
         return bench_copy(args)
     if args.workload == "channels":
         return bench_channels(args)
+    if args.workload == "output":
+        return bench_output(args)
     st, d = build_state(args.n, args.seed)
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
@@ -493,6 +495,137 @@ def serve_capacity(reqs, chunk: int, variants=None) -> dict:
             "plain_launches_scenarios_per_h": round(3600.0 * n / t_plain, 1),
             "speedup_vs_plain_launches": round(t_plain / st["wall_s"], 3),
             "bitwise_identical_to_plain": bool(same)}
+
+
+def bench_output(args) -> int:
+    """The output cadence on the headline basin (4096^2 rain on terrain): 1800 s of
+    storm with a frame (depth, u, v) every 300 s plus the baseline, written to a
+    result store, three ways -- no output, device frames written by a writer thread,
+    and the reference's synchronous read-back per output tick (run.py:389-400)."""
+    import shutil
+    import tempfile
+
+    from rbs.core.grid import Grid
+    from rbs.driver import DeviceRun
+    from rbs.io.output import OutputPipeline, run_with_output
+    from rbs.io.zarr_store import ZarrWriter, open_store
+    from rbs.scheduler import MultiRateScheduler
+
+    n = args.n
+    t0 = time.time()
+    from rbs import terrain
+
+    d = terrain.rain_on_terrain(n, dx=10.0, seed=args.seed)
+    log(f"[output] {n}x{n} inputs in {time.time() - t0:.0f} s")
+    rain_sum = float(d["rain"].astype(np.float64).sum())
+    end, every = 1800.0, 300.0
+    n_frames = int(round(end / every)) + 1
+    grid = Grid(ny=n, nx=n, dx=10.0)
+    root = Path(tempfile.mkdtemp(prefix="rbs_output_"))
+
+    def fresh():
+        from rbs.core.state import State
+
+        st = State.from_bed(d["bed"], d["dx"], depth=0.0, manning=d["manning"])
+        st.set_rain_field(d["rain"])
+        st.set_infiltration(d["infil"])
+        st.set_open_boundaries({e: "open" for e in ("north", "south", "east", "west")})
+        st.arm_source_compensation()
+        sched = MultiRateScheduler(end_time=end, output_every=every, events=[7200.0])
+        run = DeviceRun(st, "local_inertial", sched, alpha=0.7, dt_max=30.0, rain_is_field=True,
+                        rain_duration=7200.0, rain_field_sum=rain_sum)
+        return st, sched, run
+
+    results = {}
+    try:
+        # 1. no output
+        st, sched, run = fresh()
+        st.sync()
+        t0 = time.perf_counter()
+        steps = sum(run.run_until(seg.t_stop) for seg in sched.segments())
+        st.sync()
+        results["none"] = time.perf_counter() - t0
+        st.close()
+        log(f"[output] no output: {steps} steps in {results['none']:.3f} s")
+        # 2. device frames, writer thread
+        st, sched, run = fresh()
+        w = ZarrWriter(root / "async.zarr", grid, n_frames, {"dx": 10.0})
+        pipe = OutputPipeline(st, w, slots=3)
+        st.sync()
+        t0 = time.perf_counter()
+        steps_b = run_with_output(run, sched, pipe)
+        st.sync()
+        stepped = time.perf_counter() - t0
+        pipe.close()
+        results["async"] = time.perf_counter() - t0
+        results["async_stepping"] = stepped
+        w.finalize()
+        st.close()
+        log(f"[output] async frames: stepping done in {stepped:.3f} s, all {pipe.written} "
+            f"frames written at {results['async']:.3f} s")
+        # 2b. device frames taken and copied, not written (the device path alone)
+        class _Discard:
+            def append(self, *a):
+                pass
+
+        st, sched, run = fresh()
+        pipe = OutputPipeline(st, _Discard(), slots=3)
+        st.sync()
+        t0 = time.perf_counter()
+        run_with_output(run, sched, pipe)
+        st.sync()
+        pipe.close()
+        results["frames_only"] = time.perf_counter() - t0
+        st.close()
+        log(f"[output] frames captured and copied, not written: {results['frames_only']:.3f} s")
+        # 3. synchronous read-back per output tick
+        st, sched, run = fresh()
+        w = ZarrWriter(root / "sync.zarr", grid, n_frames, {"dx": 10.0})
+        st.sync()
+        t0 = time.perf_counter()
+        w.append(0.0, st.h.numpy(), *st.velocities_numpy())
+        for seg in sched.segments():
+            run.run_until(seg.t_stop)
+            if seg.is_output:
+                w.append(seg.t_stop, st.h.numpy(), *st.velocities_numpy())
+        results["sync"] = time.perf_counter() - t0
+        w.finalize()
+        st.close()
+        log(f"[output] synchronous read-back: {results['sync']:.3f} s")
+        a, b = open_store(root / "async.zarr"), open_store(root / "sync.zarr")
+        same = all(np.array_equal(a[f].frame(i), b[f].frame(i))
+                   for f in ("depth", "u", "v") for i in range(n_frames))
+        log(f"[output] async store == synchronous store: {same}")
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    frame_mb = 3 * n * n * 4 / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(n * n * steps_b / results["async"] / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps_b,
+        "warmup": 0,
+        "ms_per_step": round(results["async"] * 1e3 / steps_b, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"li_rain_output_{n}", "grid": f"{n}x{n}", "dx_m": 10.0,
+                   "frames": n_frames, "frame_fields": "depth, u, v (f32)",
+                   "frame_mb": round(frame_mb, 1), "sim_s": end, "output_every_s": every},
+        "wall_s": {k: round(v, 3) for k, v in results.items()},
+        "output_overhead_ms_per_frame": {
+            "async": round((results["async"] - results["none"]) * 1e3 / n_frames, 1),
+            "async_on_stepping": round((results["async_stepping"] - results["none"]) * 1e3
+                                       / n_frames, 1),
+            "frames_only": round((results["frames_only"] - results["none"]) * 1e3 / n_frames, 1),
+            "sync": round((results["sync"] - results["none"]) * 1e3 / n_frames, 1)},
+        "stores_identical": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 CHAN_BYTES_PER_CELL = 84  # + w, d, n_ch, the four old components, and the four new

@@ -26,7 +26,7 @@ struct FrameArgs {
   const float *hu, *hv;  // HLLC momentum, or null
   const double *dz_cum;  // bed change, or null when not requested
   const DevCtl *ctl;
-  float *depth, *u, *v, *bed;  // device slot, pitched like the state; null = not requested
+  float *depth, *u, *v, *bed;  // device slot, dense (ny, nx) planes; null = not requested
   double *t;                   // the frame's clock
   int64_t pitch;
   int32_t ny, nx;
@@ -41,8 +41,9 @@ __global__ __launch_bounds__(kThreads) void frame_kernel(FrameArgs a) {
   for (int r = blockIdx.y; r < a.ny; r += gridDim.y) {
     for (int c = blockIdx.x * kThreads + threadIdx.x; c < a.nx; c += gridDim.x * kThreads) {
       const int64_t o = (int64_t)r * P + c;
+      const int64_t od = (int64_t)r * a.nx + c;  // dense: one plane is one copy
       const float hh = h[o];
-      if (a.depth) a.depth[o] = hh;
+      if (a.depth) a.depth[od] = hh;
       if (a.u) {
         const bool wet = hh >= kHDry;  // state.py:244
         float mx, my;
@@ -55,10 +56,10 @@ __global__ __launch_bounds__(kThreads) void frame_kernel(FrameArgs a) {
           mx = 0.5f * (qx[o] + qx[o + 1]);
           my = 0.5f * (qy[o] + qy[o + P]);
         }
-        a.u[o] = wet ? mx / hh : 0.0f;
-        a.v[o] = wet ? my / hh : 0.0f;
+        a.u[od] = wet ? mx / hh : 0.0f;
+        a.v[od] = wet ? my / hh : 0.0f;
       }
-      if (a.bed) a.bed[o] = (float)a.dz_cum[o];  // zarr_writer.py:147
+      if (a.bed) a.bed[od] = (float)a.dz_cum[o];  // zarr_writer.py:147
     }
   }
 }
@@ -71,8 +72,8 @@ struct rbs_frames {
   int slots = 0;
   int nfield = 0;
   hipStream_t copy = nullptr;
-  // per slot: device staging (nfield pitched planes + the clock), pinned host
-  // buffers (nfield dense planes + the clock), events, state
+  // per slot: device staging and pinned host buffers (nfield dense planes + the
+  // clock each), events, state
   std::vector<float *> dev;
   std::vector<double *> dev_t;
   std::vector<float *> host;
@@ -145,8 +146,8 @@ int rbs_frames_create(rbs_state *s, uint32_t fields, int32_t slots, rbs_frames *
   };
   if (hipStreamCreateWithFlags(&q->copy, hipStreamNonBlocking) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipStreamCreate"));
-  const size_t plane_dev = (size_t)s->ny * s->pitch * sizeof(float);
-  const size_t plane_host = (size_t)s->ny * s->nx * sizeof(float);
+  const size_t plane_dev = (size_t)s->ny * s->nx * sizeof(float);
+  const size_t plane_host = plane_dev;
   q->dev.assign(slots, nullptr);
   q->dev_t.assign(slots, nullptr);
   q->host.assign(slots, nullptr);
@@ -223,7 +224,7 @@ static int capture_into(rbs_frames *q, int k) {
   }
   a.dz_cum = (q->fields & RBS_FRAME_BED_CHANGE) ? s->dz_cum : nullptr;
   a.ctl = s->ctl;
-  const size_t plane = (size_t)s->ny * s->pitch;
+  const size_t plane = (size_t)s->ny * s->nx;
   auto at_plane = [&](uint32_t f) -> float * {
     const int p = plane_of(q->fields, f);
     return p < 0 ? nullptr : q->dev[k] + (size_t)p * plane;
@@ -243,11 +244,9 @@ static int capture_into(rbs_frames *q, int k) {
   RBS_HIP(hipEventRecord(q->formed[k], s->stream));
   // the copy stream waits for the frame, not for the steps enqueued after it
   RBS_HIP(hipStreamWaitEvent(q->copy, q->formed[k], 0));
-  const size_t row = (size_t)s->nx * sizeof(float);
-  for (int p = 0; p < q->nfield; ++p)
-    RBS_HIP(hipMemcpy2DAsync((char *)q->host[k] + (size_t)p * s->ny * row, row,
-                             q->dev[k] + (size_t)p * plane, (size_t)s->pitch * sizeof(float), row,
-                             s->ny, hipMemcpyDeviceToHost, q->copy));
+  // one 1-D copy of all planes (a 2-D copy would move row by row)
+  RBS_HIP(hipMemcpyAsync(q->host[k], q->dev[k], plane * sizeof(float) * q->nfield,
+                         hipMemcpyDeviceToHost, q->copy));
   RBS_HIP(hipMemcpyAsync(q->host_t[k], q->dev_t[k], sizeof(double), hipMemcpyDeviceToHost,
                          q->copy));
   RBS_HIP(hipEventRecord(q->copied[k], q->copy));

@@ -176,6 +176,9 @@ struct rbs_state {
   float *qx_ch[2] = {nullptr, nullptr}, *qx_fp[2] = {nullptr, nullptr};
   float *qy_ch[2] = {nullptr, nullptr}, *qy_fp[2] = {nullptr, nullptr};
   int32_t chan_variant = 0;
+  // dense device staging for field transfers (one block of rows at a time)
+  void *xfer = nullptr;
+  size_t xfer_cap = 0;
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;

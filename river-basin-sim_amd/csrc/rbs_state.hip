@@ -333,7 +333,7 @@ static int state_destroy(rbs_state *s) {
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
                   s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol, s->ch_w, s->ch_d,
                   s->ch_n, s->qx_ch[0], s->qx_ch[1], s->qx_fp[0], s->qx_fp[1], s->qy_ch[0],
-                  s->qy_ch[1], s->qy_fp[0], s->qy_fp[1]};
+                  s->qy_ch[1], s->qy_fp[0], s->qy_fp[1], s->xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &c : s->stage) {
@@ -369,24 +369,62 @@ int rbs_state_bytes(const rbs_state *s, uint64_t *bytes) {
 
 // Host<->device field copies move in blocks of rows of at most kCopyChunk bytes,
 // each under the API lock on its own, so a graph capture on another thread never
-// waits for more than one block.
+// waits for more than one block.  A pitched 2-D copy moves row by row (one DMA per
+// row); instead each block is packed to (or unpacked from) a dense device staging
+// block by a kernel and crosses the bus as one 1-D copy.
 constexpr size_t kCopyChunk = size_t(4) << 20;
+
+// rows x row_w 4-byte words between a pitched field and a dense block.
+__global__ void pack_rows_kernel(const uint32_t *__restrict__ src, int64_t pitch_w,
+                                 uint32_t *__restrict__ dst, int64_t row_w, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / row_w, k = i - r * row_w;
+    dst[i] = src[r * pitch_w + k];
+  }
+}
+__global__ void unpack_rows_kernel(const uint32_t *__restrict__ src, int64_t pitch_w,
+                                   uint32_t *__restrict__ dst, int64_t row_w, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / row_w, k = i - r * row_w;
+    dst[r * pitch_w + k] = src[i];
+  }
+}
 
 static int copy_rows(rbs_state *s, const FieldView &v, void *host, bool to_device) {
   const size_t row_bytes = (size_t)v.cols * v.elem;
   const size_t dpitch = (size_t)s->pitch * v.elem;
   const int rows_per = (int)std::max<size_t>(1, kCopyChunk / row_bytes);
+  const size_t need = std::max(kCopyChunk, row_bytes);
   for (int r0 = 0; r0 < v.rows; r0 += rows_per) {
     const int nr = std::min(rows_per, v.rows - r0);
     char *h = (char *)host + (size_t)r0 * row_bytes;
     char *d = (char *)v.dev + (size_t)r0 * dpitch;
     RBS_API_SHARED;
-    if (to_device)
-      RBS_HIP(hipMemcpy2DAsync(d, dpitch, h, row_bytes, row_bytes, nr, hipMemcpyHostToDevice,
-                               s->stream));
-    else
-      RBS_HIP(hipMemcpy2DAsync(h, row_bytes, d, dpitch, row_bytes, nr, hipMemcpyDeviceToHost,
-                               s->stream));
+    if (s->xfer_cap < need) {
+      if (s->xfer) RBS_HIP(hipFree(s->xfer));
+      s->xfer = nullptr;
+      s->xfer_cap = 0;
+      RBS_HIP(hipMalloc(&s->xfer, need));
+      s->xfer_cap = need;
+    }
+    const int64_t row_w = (int64_t)(row_bytes / 4), pitch_w = (int64_t)(dpitch / 4);
+    const int64_t n = row_w * nr;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    if (to_device) {
+      RBS_HIP(hipMemcpyAsync(s->xfer, h, (size_t)nr * row_bytes, hipMemcpyHostToDevice,
+                             s->stream));
+      hipLaunchKernelGGL(unpack_rows_kernel, dim3(grid), dim3(256), 0, s->stream,
+                         (const uint32_t *)s->xfer, pitch_w, (uint32_t *)d, row_w, n);
+      RBS_HIP(hipGetLastError());
+    } else {
+      hipLaunchKernelGGL(pack_rows_kernel, dim3(grid), dim3(256), 0, s->stream,
+                         (const uint32_t *)d, pitch_w, (uint32_t *)s->xfer, row_w, n);
+      RBS_HIP(hipGetLastError());
+      RBS_HIP(hipMemcpyAsync(h, s->xfer, (size_t)nr * row_bytes, hipMemcpyDeviceToHost,
+                             s->stream));
+    }
     RBS_HIP(hipStreamSynchronize(s->stream));
   }
   return RBS_OK;

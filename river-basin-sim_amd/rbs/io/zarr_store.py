@@ -78,17 +78,23 @@ class _Array:
         })
 
     def write_chunk(self, index: tuple[int, ...], block: np.ndarray) -> None:
-        """Write one whole chunk (edge chunks are padded to the chunk shape)."""
-        full = np.zeros(self.chunks, dtype=self.dtype)
-        block = np.asarray(block, dtype=self.dtype).reshape(
-            tuple(min(c, s - i * c) for c, s, i in zip(self.chunks, self.shape, index)))
-        full[tuple(slice(0, n) for n in block.shape)] = block
-        data = full.tobytes(order="C")
+        """Write one whole chunk (edge chunks are padded to the chunk shape).  A block
+        that already is a contiguous little-endian chunk is written from its own
+        buffer, without a copy."""
+        inner = tuple(min(c, s - i * c) for c, s, i in zip(self.chunks, self.shape, index))
+        block = np.asarray(block)
+        if inner == self.chunks and block.dtype == self.dtype and block.flags["C_CONTIGUOUS"]:
+            data = memoryview(block.reshape(self.chunks)).cast("B")
+        else:
+            full = np.zeros(self.chunks, dtype=self.dtype)
+            full[tuple(slice(0, n) for n in inner)] = np.asarray(block, dtype=self.dtype).reshape(inner)
+            data = memoryview(full).cast("B")
         if self.compress:
             data = gzip.compress(data, compresslevel=1, mtime=0)
         key = self.path.joinpath("c", *[str(i) for i in index])
         key.parent.mkdir(parents=True, exist_ok=True)
-        key.write_bytes(data)
+        with open(key, "wb") as f:
+            f.write(data)
 
     def write_all(self, values: np.ndarray) -> None:
         values = np.asarray(values, dtype=self.dtype)
@@ -106,8 +112,13 @@ class ZarrWriter:
     ``finalize``; ``bed_change`` exists exactly when the run has morphology."""
 
     def __init__(self, path, grid, n_frames: int, attrs: dict, *, bed_change: bool = False,
-                 compress: bool = False):
+                 compress: bool = False, parallel: bool = True):
         self.path = Path(path)
+        self._pool = None
+        if parallel:
+            from concurrent.futures import ThreadPoolExecutor
+
+            self._pool = ThreadPoolExecutor(max_workers=4, thread_name_prefix="rbs-store")
         self.grid = grid
         self.n_frames = int(n_frames)
         self._i = 0
@@ -155,11 +166,15 @@ class ZarrWriter:
                 f"frame: {'yes' if bed_change is not None else 'no'}); an omitted "
                 "frame would store zeros, which reads as 'the bed did not move'")
         i = self._i
-        self._depth.write_chunk((i, 0, 0), depth)
-        self._u.write_chunk((i, 0, 0), u)
-        self._v.write_chunk((i, 0, 0), v)
+        jobs = [(self._depth, depth), (self._u, u), (self._v, v)]
         if self._bed_change is not None:
-            self._bed_change.write_chunk((i, 0, 0), np.asarray(bed_change, dtype=np.float32))
+            jobs.append((self._bed_change, np.asarray(bed_change, dtype=np.float32)))
+        if self._pool is not None:  # file writes release the GIL: the fields go side by side
+            for fut in [self._pool.submit(a.write_chunk, (i, 0, 0), x) for a, x in jobs]:
+                fut.result()
+        else:
+            for a, x in jobs:
+                a.write_chunk((i, 0, 0), x)
         self._times[i] = float(time)
         self._time.write_chunk((0,), self._times)
         self._i += 1
@@ -176,6 +191,9 @@ class ZarrWriter:
         if self._i < self.n_frames:
             self.attrs["n_frames_allocated"] = self.n_frames
         self._write_group()
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
 
 
 # --- reading --------------------------------------------------------------------------
