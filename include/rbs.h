@@ -110,6 +110,7 @@ typedef struct rbs_ledger {
   int64_t steps;            /* steps taken                                     */
   float speed_max;          /* LI: max h; HLLC: max |u|,|v| + sqrt(g h)        */
   int32_t active;           /* device-driven run: 1 while t < t_stop           */
+  double release_out, release_out_c; /* reservoir delivery rounding, banked as a sink */
 } rbs_ledger;
 
 /* Forcing a device-driven run evaluates per step (run.py:351-388). */
@@ -225,6 +226,34 @@ int rbs_frames_wait(rbs_frames *q, int32_t slot, double *t, float **depth, float
 int rbs_frames_release(rbs_frames *q, int32_t slot);
 /* Device bytes held by the queue (pinned host bytes are the dense equivalent). */
 int rbs_frames_bytes(const rbs_frames *q, uint64_t *bytes);
+
+/* ---- reservoirs (processes/reservoir.py:60-275, io/config.py:147-255) ------------- */
+/* One dam's release rule and geometry.  Boxes are inclusive (row0, col0, row1, col1);
+ * the gauge is the pool's lowest-bed cell, fixed when the operator is built. */
+#define RBS_RULE_FIXED 1          /* a constant release_m3_s                         */
+#define RBS_RULE_TARGET_STAGE 2   /* 0 at target_stage_m rising to release_max_m3_s at the crest */
+typedef struct rbs_reservoir {
+  int32_t pool[4];
+  int32_t outlet[4];
+  int32_t gauge_row, gauge_col;
+  int32_t rule;
+  int32_t pad;
+  double release_m3_s;
+  double target_stage_m;
+  double crest_m;
+  double release_max_m3_s;
+} rbs_reservoir;
+/* One activation (reservoir.py:133-147); stage is NaN when the gauge cell is dry. */
+typedef struct rbs_release {
+  double time, stage, discharge_m3_s, volume_m3;
+} rbs_release;
+/* ReservoirOperator.advance (reservoir.py:219-270) as one kernel on the state's stream:
+ * read the gauge stage, evaluate the rule, move min(Q dt_slow, pool volume) from the
+ * pool (in proportion to depth, each cell's exact float32 change summed in float64)
+ * to the outlet box (evenly, the float32 rounding of each cell banked into the
+ * ledger's release_out).  Fills *out; the state's CFL reduction is marked stale. */
+int rbs_reservoir_release(rbs_state *st, const rbs_reservoir *r, double t, double dt_slow,
+                          rbs_release *out);
 
 /* ---- ledger and device-driven runs -------------------------------------------------- */
 /* MassLedger inputs: device volume, sink totals, clock (massbalance.py:109-135). */

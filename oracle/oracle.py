@@ -367,6 +367,52 @@ def fill_span(span: float, dt_raw: float) -> float:
     return float(lib().orc_fill_span(span, dt_raw))
 
 
+# --- reservoirs (processes/reservoir.py:83-270, io/config.py:239-255) --------------------
+def release_rule(rule: dict, stage):
+    """Structure.discharge_at: ``rule`` holds release_rule and its parameters."""
+    if rule["release_rule"] == "none" or stage is None:
+        return 0.0
+    if rule["release_rule"] == "fixed":
+        return float(rule["release_m3_s"])
+    span = rule["crest_m"] - float(rule["target_stage_m"])
+    frac = (stage - float(rule["target_stage_m"])) / span
+    return float(rule["release_max_m3_s"]) * min(max(frac, 0.0), 1.0)
+
+
+def reservoir_release(st: State, rule: dict, pool, outlet, gauge, t: float, dt_slow: float):
+    """ReservoirOperator.advance over an oracle state: the gauge stage, the rule, then
+    _withdraw_pool and _deliver_release in float32 with float64 sums, the delivery's
+    rounding banked into loss_cum.  Returns (time, stage, discharge, volume)."""
+    st._ensure_loss_cum()
+    gi, gj = gauge
+    h = float(st.h[gi, gj])
+    stage = None if h <= H_DRY else float(st.z[gi, gj]) + h
+    q = release_rule(rule, stage)
+    volume = 0.0
+    if q > 0.0:
+        area = st.cell_area
+        r0, c0, r1, c1 = pool
+        ps = (slice(r0, r1 + 1), slice(c0, c1 + 1))
+        stored = float(st.h[ps].astype(np.float64).sum()) * area
+        if stored > 0.0:
+            scale = np.float32(min(1.0, (q * float(dt_slow)) / stored))
+            avail = st.h[ps].copy()
+            h_new = avail - avail * scale
+            st.h[ps] = h_new
+            removed = float((avail.astype(np.float64) - h_new.astype(np.float64)).sum()) * area
+            if removed > 0.0:
+                o0, p0, o1, p1 = outlet
+                os_ = (slice(o0, o1 + 1), slice(p0, p1 + 1))
+                n_out = (o1 - o0 + 1) * (p1 - p0 + 1)
+                add = removed / area / n_out
+                av = st.h[os_].copy()
+                hn = av + np.float32(add)
+                st.h[os_] = hn
+                st.loss_cum[os_] += add - (hn.astype(np.float64) - av.astype(np.float64))
+                volume = removed
+    return float(t), stage, q, volume
+
+
 # --- host-side reference pieces ---------------------------------------------------
 def stage_at(curve, t: float) -> float:
     """boundaries.py:49-70 -- piecewise linear, held outside its range."""

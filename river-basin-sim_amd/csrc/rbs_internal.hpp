@@ -73,6 +73,7 @@ struct DevCtl {
   double point_in, point_in_c;
   double infil_out, infil_out_c;
   double edge_out, edge_out_c;
+  double release_out, release_out_c;  // reservoir delivery rounding, banked
   // device-driven run parameters (copied from rbs_run_params)
   int32_t run_scheme, run_limit, run_field, n_sync;
   double alpha, dt_max, rain_m_s, rain_duration, rain_field_sum;
@@ -179,6 +180,7 @@ struct rbs_state {
   // dense device staging for field transfers (one block of rows at a time)
   void *xfer = nullptr;
   size_t xfer_cap = 0;
+  rbs_release *resv_rec = nullptr;  // device record of the last reservoir release
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;

@@ -42,6 +42,7 @@ __global__ void clock_reset_kernel(DevCtl *c, double t) {
   c->point_in = c->point_in_c = 0.0;
   c->infil_out = c->infil_out_c = 0.0;
   c->edge_out = c->edge_out_c = 0.0;
+  c->release_out = c->release_out_c = 0.0;
 }
 
 __global__ void stage_config_kernel(DevCtl *c, int n0, int n1, int n2, int n3, const double *t0,
@@ -333,7 +334,7 @@ static int state_destroy(rbs_state *s) {
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
                   s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol, s->ch_w, s->ch_d,
                   s->ch_n, s->qx_ch[0], s->qx_ch[1], s->qx_fp[0], s->qx_fp[1], s->qy_ch[0],
-                  s->qy_ch[1], s->qy_fp[0], s->qy_fp[1], s->xfer};
+                  s->qy_ch[1], s->qy_fp[0], s->qy_fp[1], s->xfer, s->resv_rec};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &c : s->stage) {
@@ -647,6 +648,8 @@ int rbs_ledger_read(rbs_state *s, rbs_ledger *out) {
   out->infil_out_c = c.infil_out_c;
   out->edge_out = c.edge_out;
   out->edge_out_c = c.edge_out_c;
+  out->release_out = c.release_out;
+  out->release_out_c = c.release_out_c;
   out->t = c.t;
   out->dt = c.dt;
   out->steps = c.steps;

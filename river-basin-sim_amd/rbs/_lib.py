@@ -48,7 +48,29 @@ class Ledger(ctypes.Structure):
         ("steps", ctypes.c_int64),
         ("speed_max", ctypes.c_float),
         ("active", ctypes.c_int32),
+        ("release_out", ctypes.c_double), ("release_out_c", ctypes.c_double),
     ]  # fmt: skip
+
+
+class Reservoir(ctypes.Structure):
+    _fields_ = [
+        ("pool", ctypes.c_int32 * 4),
+        ("outlet", ctypes.c_int32 * 4),
+        ("gauge_row", ctypes.c_int32), ("gauge_col", ctypes.c_int32),
+        ("rule", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("release_m3_s", ctypes.c_double),
+        ("target_stage_m", ctypes.c_double),
+        ("crest_m", ctypes.c_double),
+        ("release_max_m3_s", ctypes.c_double),
+    ]  # fmt: skip
+
+
+class Release(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("stage", ctypes.c_double),
+                ("discharge_m3_s", ctypes.c_double), ("volume_m3", ctypes.c_double)]
+
+
+RULE_FIXED, RULE_TARGET_STAGE = 1, 2
 
 
 class RunParams(ctypes.Structure):
@@ -114,6 +136,8 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_frames_wait": (ctypes.c_int, [_vp, _i32, _pdbl] + [ctypes.POINTER(ctypes.POINTER(ctypes.c_float))] * 4),
     "rbs_frames_release": (ctypes.c_int, [_vp, _i32]),
     "rbs_frames_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "rbs_reservoir_release": (ctypes.c_int, [_vp, ctypes.POINTER(Reservoir), _dbl, _dbl,
+                                              ctypes.POINTER(Release)]),
     "rbs_set_chan_variant": (ctypes.c_int, [_vp, _i32]),
     "rbs_chan_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_chan_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),

@@ -59,7 +59,7 @@ class MassLedger:
         ledger = cls(cell_area=state.grid.cell_area, v0=led.volume)
         # Device flow totals at construction are the zero of this ledger.
         ledger._device_in0 = led.rain_in + led.point_in
-        ledger._device_out0 = led.infil_out + led.edge_out
+        ledger._device_out0 = led.infil_out + led.edge_out + led.release_out
         ledger.record(state, 0.0)
         return ledger
 
@@ -76,7 +76,8 @@ class MassLedger:
         led = state.ledger()
         v = led.volume
         inflow = self._inflow.total + (led.rain_in + led.point_in - self._device_in0)
-        outflow = self._outflow.total + (led.infil_out + led.edge_out - self._device_out0)
+        outflow = self._outflow.total + (led.infil_out + led.edge_out + led.release_out
+                                         - self._device_out0)
         residual = inflow - outflow - (v - self.v0)
         self._peak_v = max(self._peak_v, v)
         denom = max(abs(inflow), abs(v), self._peak_v, 1e-12)

@@ -218,10 +218,11 @@ class State:
         """Sinks bank into the device ledger, which always exists (no-op)."""
 
     def loss_volume(self, cell_area: float | None = None) -> float:
-        """Cumulative sink volume (m^3): infiltration + edge drains, float64 on device."""
+        """Cumulative sink volume (m^3): infiltration, edge drains and the banked
+        rounding of reservoir deliveries, float64 on device."""
         led = self.ledger()
         area_scale = 1.0 if cell_area is None else cell_area / self.grid.cell_area
-        return (led.infil_out + led.edge_out) * area_scale
+        return (led.infil_out + led.edge_out + led.release_out) * area_scale
 
     def ledger(self) -> _lib.Ledger:
         out = _lib.Ledger()
