@@ -180,7 +180,7 @@ struct rbs_state {
   // dense device staging for field transfers (one block of rows at a time)
   void *xfer = nullptr;
   size_t xfer_cap = 0;
-  rbs_release *resv_rec = nullptr;  // device record of the last reservoir release
+  rbs_release *resv_rec = nullptr;  // reservoir releases: record, scratch, block partials
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;

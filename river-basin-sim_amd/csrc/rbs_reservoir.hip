@@ -9,15 +9,19 @@
 //     float64 difference between that share and what the float32 addition landed is
 //     banked as a sink.
 // The reference does this in three host round trips (the gauge read, the pool sum,
-// the sum of what left).  Here one block of 1024 threads does it all on the device,
-// with fixed-order float64 block sums; only the 32-byte record comes back.
+// the sum of what left).  Here four launches on the state's stream do it on the
+// device: the pool sum and the withdrawal over many blocks (the pool's rows dealt
+// round-robin, per-block partials), the scale and the delivery in one block each, all
+// sums float64 in a fixed order; only the 32-byte record comes back.
 #include <cmath>
 
 #include "rbs_device.hpp"
 
 namespace rbs {
 
-constexpr int kResvThreads = 1024;
+constexpr int kResvThreads = 256;   // the pool passes: many blocks of one row stride
+constexpr int kResvBlocks = 2048;   // at most; the pool's rows are dealt round-robin
+constexpr int kDeliverThreads = 1024;
 
 struct ResvArgs {
   float *h;
@@ -26,19 +30,25 @@ struct ResvArgs {
   rbs_reservoir r;
   double t, dt_slow, area;
   int64_t pitch;
-  rbs_release *rec;
 };
 
-// Sum of one double per thread, the same order every launch; the result is returned
-// to every thread.
+// What the launches of one release hand on to each other.
+struct ResvScratch {
+  double stage, q;
+  float scale;
+  int32_t go;  // 1: the pool holds water and the rule asks for some
+};
+
+// Fixed-order sum of one double per thread over a block of NT; every thread gets it.
+template <int NT>
 __device__ double block_sum(double v, double *s_part) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();  // s_part may still be read by the previous sum
+  __syncthreads();  // s_part may still be read by a previous sum
   if (lane == 0) s_part[wid] = v;
   __syncthreads();
   double total = 0.0;
-  for (int w = 0; w < kResvThreads / 64; ++w) total += s_part[w];
+  for (int w = 0; w < NT / 64; ++w) total += s_part[w];
   return total;
 }
 
@@ -53,68 +63,111 @@ __device__ double release_rule(const rbs_reservoir &r, double stage) {
   return r.release_max_m3_s * (1.0 < lo ? 1.0 : lo);  // and min(lo, 1.0)
 }
 
-__global__ __launch_bounds__(kResvThreads) void reservoir_kernel(ResvArgs a) {
+// pool_stage (reservoir.py:198-217): z + h at the gauge, None (NaN) when h <= H_DRY.
+__device__ double gauge_stage(const ResvArgs &a) {
+  const int64_t og = (int64_t)a.r.gauge_row * a.pitch + a.r.gauge_col;
+  const double hg = (double)a.h[og];
+  return hg <= kHDry64 ? NAN : (double)a.z[og] + hg;
+}
+
+// 1. Partial sums of f64(h) over the pool, the pool's rows dealt round-robin to blocks.
+__global__ __launch_bounds__(kResvThreads) void resv_sum_kernel(ResvArgs a, double *part) {
   __shared__ double s_part[kResvThreads / 64];
   const rbs_reservoir &r = a.r;
-  const int64_t P = a.pitch;
-  const int tid = threadIdx.x;
-  const int pr = r.pool[2] - r.pool[0] + 1, pc = r.pool[3] - r.pool[1] + 1;
-  const int orr = r.outlet[2] - r.outlet[0] + 1, oc = r.outlet[3] - r.outlet[1] + 1;
-  const int64_t n_pool = (int64_t)pr * pc, n_out = (int64_t)orr * oc;
-  auto pool_at = [&](int64_t k) {
-    const int64_t i = k / pc, j = k - i * pc;
-    return (r.pool[0] + i) * P + (r.pool[1] + j);
-  };
-  auto out_at = [&](int64_t k) {
-    const int64_t i = k / oc, j = k - i * oc;
-    return (r.outlet[0] + i) * P + (r.outlet[1] + j);
-  };
+  double s = 0.0;
+  if (release_rule(r, gauge_stage(a)) > 0.0) {  // block-uniform
+    for (int i = r.pool[0] + blockIdx.x; i <= r.pool[2]; i += gridDim.x) {
+      const float *row = a.h + (int64_t)i * a.pitch;
+      for (int j = r.pool[1] + threadIdx.x; j <= r.pool[3]; j += kResvThreads) s += (double)row[j];
+    }
+  }
+  s = block_sum<kResvThreads>(s, s_part);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
 
-  // pool_stage (reservoir.py:198-217): z + h at the gauge, None (NaN) when h <= H_DRY
-  const int64_t og = (int64_t)r.gauge_row * P + r.gauge_col;
-  const double hg = (double)a.h[og];
-  const double stage = hg <= kHDry64 ? NAN : (double)a.z[og] + hg;
-  const double q = release_rule(r, stage);
-  double volume = 0.0;
-  if (q > 0.0) {  // block-uniform
-    const double requested = q * a.dt_slow;
-    double s = 0.0;
-    for (int64_t k = tid; k < n_pool; k += kResvThreads) s += (double)a.h[pool_at(k)];
-    const double stored = block_sum(s, s_part) * a.area;
-    if (stored > 0.0) {
-      const double x = requested / stored;
-      const float scale = (float)(x < 1.0 ? x : 1.0);  // min(1.0, x), then float32
-      double taken = 0.0;
-      for (int64_t k = tid; k < n_pool; k += kResvThreads) {
-        const int64_t o = pool_at(k);
-        const float avail = a.h[o];
+// 2. One block: the stage, the rule, the pool volume and the withdrawal factor.
+__global__ __launch_bounds__(kResvThreads) void resv_scale_kernel(ResvArgs a, const double *part,
+                                                                  int nb, ResvScratch *sc) {
+  __shared__ double s_part[kResvThreads / 64];
+  const double stage = gauge_stage(a);
+  const double q = release_rule(a.r, stage);
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nb; k += kResvThreads) s += part[k];
+  const double stored = block_sum<kResvThreads>(s, s_part) * a.area;
+  if (threadIdx.x == 0) {
+    sc->stage = stage;
+    sc->q = q;
+    sc->go = 0;
+    sc->scale = 0.0f;
+    if (q > 0.0 && stored > 0.0) {
+      const double x = (q * a.dt_slow) / stored;
+      sc->scale = (float)(x < 1.0 ? x : 1.0);  // min(1.0, x), then float32
+      sc->go = 1;
+    }
+  }
+}
+
+// 3. _withdraw_pool (:83-105): h -= h * scale per pool cell, partial sums of what left.
+__global__ __launch_bounds__(kResvThreads) void resv_withdraw_kernel(ResvArgs a,
+                                                                     const ResvScratch *sc,
+                                                                     double *part) {
+  __shared__ double s_part[kResvThreads / 64];
+  const rbs_reservoir &r = a.r;
+  double taken = 0.0;
+  if (sc->go) {
+    const float scale = sc->scale;
+    for (int i = r.pool[0] + blockIdx.x; i <= r.pool[2]; i += gridDim.x) {
+      float *row = a.h + (int64_t)i * a.pitch;
+      for (int j = r.pool[1] + threadIdx.x; j <= r.pool[3]; j += kResvThreads) {
+        const float avail = row[j];
         const float h_new = avail - avail * scale;
-        a.h[o] = h_new;
+        row[j] = h_new;
         taken += (double)avail - (double)h_new;
-      }
-      const double removed = block_sum(taken, s_part) * a.area;
-      if (removed > 0.0) {
-        const double add = removed / a.area / (double)n_out;
-        const float add32 = (float)add;
-        double shortfall = 0.0;
-        for (int64_t k = tid; k < n_out; k += kResvThreads) {
-          const int64_t o = out_at(k);
-          const float avail = a.h[o];
-          const float h_new = avail + add32;
-          a.h[o] = h_new;
-          shortfall += add - ((double)h_new - (double)avail);
-        }
-        const double banked = block_sum(shortfall, s_part);
-        if (tid == 0) kahan_add64(a.ctl->release_out, a.ctl->release_out_c, banked * a.area);
-        volume = removed;
       }
     }
   }
-  if (tid == 0) {
-    a.rec->time = a.t;
-    a.rec->stage = stage;
-    a.rec->discharge_m3_s = q;
-    a.rec->volume_m3 = volume;
+  taken = block_sum<kResvThreads>(taken, s_part);
+  if (threadIdx.x == 0) part[blockIdx.x] = taken;
+}
+
+// 4. One block: what left the pool, _deliver_release (:108-130) over the outlet with
+// its rounding banked, and the record.
+__global__ __launch_bounds__(kDeliverThreads) void resv_deliver_kernel(ResvArgs a,
+                                                                       const double *part, int nb,
+                                                                       const ResvScratch *sc,
+                                                                       rbs_release *rec) {
+  __shared__ double s_part[kDeliverThreads / 64];
+  const rbs_reservoir &r = a.r;
+  double volume = 0.0;
+  if (sc->go) {  // block-uniform
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kDeliverThreads) s += part[k];
+    const double removed = block_sum<kDeliverThreads>(s, s_part) * a.area;
+    if (removed > 0.0) {
+      const int oc = r.outlet[3] - r.outlet[1] + 1;
+      const int64_t n_out = (int64_t)(r.outlet[2] - r.outlet[0] + 1) * oc;
+      const double add = removed / a.area / (double)n_out;
+      const float add32 = (float)add;
+      double shortfall = 0.0;
+      for (int i = r.outlet[0]; i <= r.outlet[2]; ++i) {
+        float *row = a.h + (int64_t)i * a.pitch;
+        for (int j = r.outlet[1] + threadIdx.x; j <= r.outlet[3]; j += kDeliverThreads) {
+          const float avail = row[j];
+          const float h_new = avail + add32;
+          row[j] = h_new;
+          shortfall += add - ((double)h_new - (double)avail);
+        }
+      }
+      const double banked = block_sum<kDeliverThreads>(shortfall, s_part);
+      if (threadIdx.x == 0) kahan_add64(a.ctl->release_out, a.ctl->release_out_c, banked * a.area);
+      volume = removed;
+    }
+  }
+  if (threadIdx.x == 0) {
+    rec->time = a.t;
+    rec->stage = sc->stage;
+    rec->discharge_m3_s = sc->q;
+    rec->volume_m3 = volume;
   }
 }
 
@@ -141,8 +194,14 @@ extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, doubl
   if (r->rule != RBS_RULE_FIXED && r->rule != RBS_RULE_TARGET_STAGE)
     return set_error("unknown release rule %d", r->rule), RBS_E_ARG;
   if (!(dt_slow >= 0.0) || !std::isfinite(dt_slow)) return set_error("bad dt_slow %g", dt_slow), RBS_E_ARG;
-  if (!s->resv_rec) RBS_HIP(hipMalloc(&s->resv_rec, sizeof(rbs_release)));
+  if (!s->resv_rec) {
+    // the record, the scratch and the block partials, kept for the state's life
+    const size_t bytes = sizeof(rbs_release) + sizeof(ResvScratch) + sizeof(double) * kResvBlocks;
+    RBS_HIP(hipMalloc(&s->resv_rec, bytes));
+  }
   rbs_release *d_rec = s->resv_rec;
+  ResvScratch *sc = reinterpret_cast<ResvScratch *>(d_rec + 1);
+  double *part = reinterpret_cast<double *>(sc + 1);
   ResvArgs a{};
   a.h = s->h[s->cur];
   a.z = s->z;
@@ -152,8 +211,13 @@ extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, doubl
   a.dt_slow = dt_slow;
   a.area = s->dx * s->dx;
   a.pitch = s->pitch;
-  a.rec = d_rec;
-  hipLaunchKernelGGL(reservoir_kernel, dim3(1), dim3(kResvThreads), 0, s->stream, a);
+  const int pool_rows = r->pool[2] - r->pool[0] + 1;
+  const int nb = pool_rows < kResvBlocks ? pool_rows : kResvBlocks;
+  hipLaunchKernelGGL(resv_sum_kernel, dim3(nb), dim3(kResvThreads), 0, s->stream, a, part);
+  hipLaunchKernelGGL(resv_scale_kernel, dim3(1), dim3(kResvThreads), 0, s->stream, a, part, nb, sc);
+  hipLaunchKernelGGL(resv_withdraw_kernel, dim3(nb), dim3(kResvThreads), 0, s->stream, a, sc, part);
+  hipLaunchKernelGGL(resv_deliver_kernel, dim3(1), dim3(kDeliverThreads), 0, s->stream, a, part,
+                     nb, sc, d_rec);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = hipMemcpyAsync(out, d_rec, sizeof(rbs_release), hipMemcpyDeviceToHost, s->stream);
