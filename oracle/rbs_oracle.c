@@ -1045,6 +1045,38 @@ static void hl_accumulate(orc_state *s, const float *h, float dx) {
 }
 
 /* hllc.py:1152-1269 _eval_L */
+/* The bedload integral of an HLLC stage (this build's coupling; the reference runs
+ * morphology on the local-inertial scheme only, config.py:451-461): every interior
+ * face adds 0.5*dt times the MPM capacity of its limited mass flux at the LI face
+ * depth max(eta) - max(z) of the stage state (sediment.py:161-273), so the two
+ * stages average the transport rate as SSP-RK2 averages the flux. */
+static void hl_transport(orc_state *s, float dt) {
+  const int ny = s->ny, nx = s->nx, w = nx + 1;
+  const float hdt = 0.5f * dt;
+  if (nx > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ny; ++i)
+      for (int j = 1; j < nx; ++j) {
+        const size_t a = IX(i, j - 1, nx), b = IX(i, j, nx), f = IX(i, j, w);
+        float hf = fmx(s->eta[a], s->eta[b]) - fmx(s->z[a], s->z[b]);
+        float qs = orc_face_capacity(s->fx_h[f], hf, hf, 0.5f * (s->n[a] + s->n[b]),
+                                     0.5f * (s->d50[a] + s->d50[b]));
+        kahan_add(&s->qs_int_x[f], &s->qs_comp_x[f], qs * hdt);
+      }
+  }
+  if (ny > 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < ny; ++i)
+      for (int j = 0; j < nx; ++j) {
+        const size_t a = IX(i - 1, j, nx), b = IX(i, j, nx), f = IX(i, j, nx);
+        float hf = fmx(s->eta[a], s->eta[b]) - fmx(s->z[a], s->z[b]);
+        float qs = orc_face_capacity(s->fy_h[f], hf, hf, 0.5f * (s->n[a] + s->n[b]),
+                                     0.5f * (s->d50[a] + s->d50[b]));
+        kahan_add(&s->qs_int_y[f], &s->qs_comp_y[f], qs * hdt);
+      }
+  }
+}
+
 static void hl_eval_L(orc_state *s, const float *h, const float *hu, const float *hv,
                       float dt, const float stages[4]) {
   const float dxf = (float)s->dx;
@@ -1053,6 +1085,7 @@ static void hl_eval_L(orc_state *s, const float *h, const float *hu, const float
   hl_flux_y(s, h);
   hl_apply_boundaries(s, h, stages);
   hl_mass_limit(s, h, dxf, dt);
+  if (s->d50) hl_transport(s, dt);
   hl_accumulate(s, h, dxf);
 }
 

@@ -11,6 +11,11 @@
 //   stage 1: Euler predictor U1                       (_rk_stage1 :850-878)
 //   stage 2: Heun corrector in place, then friction, sources and the CFL
 //            wave-speed partials                       (:881-990, :1338-1373)
+//   with the bedload integral armed: each stage adds 0.5*dt times the MPM capacity
+//   of every interior face's limited mass flux (the LI law and face depth,
+//   sediment.py:161-273), the RK2 average of the transport rate.  The reference
+//   runs morphology on the local-inertial scheme only (config.py:451-461); this
+//   coupling is this build's.
 // A tile stages h, eta, z, u, v three cells out in LDS (the 4-point MUSCL stencil
 // of the ring-1 cells' faces), computes every face of the ring-1 cells (their
 // mass fluxes feed the donor beta the tile's own faces need), then works on its
@@ -27,6 +32,8 @@ struct HlArgs {
   const float *z, *n;
   float *h_comp;
   const float *rain, *infil;
+  const float *d50;
+  float *qs_int_x, *qs_comp_x, *qs_int_y, *qs_comp_y;
   const DevCtl *ctl;
   Partials part;
   double dx64;
@@ -36,7 +43,7 @@ struct HlArgs {
   int32_t flags;  // HL_* below
 };
 
-enum : int { HL_COMP = 1, HL_FIELD = 2, HL_INFIL = 4, HL_LOSS = 8 };
+enum : int { HL_COMP = 1, HL_FIELD = 2, HL_INFIL = 4, HL_LOSS = 8, HL_SED = 16 };
 
 // hllc.py:158-165 _minmod
 __device__ __forceinline__ float minmod(float a, float b) {
@@ -357,6 +364,28 @@ __global__ __launch_bounds__(kThreads) void hllc_stage_kernel(HlArgs a) {
     const int r = y0 + lr, c = x0 + lc;
     if (k < BY * BX && r < ny && c < nx) {
       const int er = lr + 3, ec = lc + 3;
+      const int64_t o = (int64_t)r * P + c;
+      if (a.flags & HL_SED) {  // own interior faces: left (c >= 1), top (r >= 1)
+        const float hdt = 0.5f * dt;
+        if (c >= 1) {
+          const float hf = fmx(s_e[er][ec - 1], s_e[er][ec]) - fmx(s_z[er][ec - 1], s_z[er][ec]);
+          const float qs = face_capacity(s_fxh[lr + 1][lc + 1], hf, hf, 0.5f * (a.n[o - 1] + a.n[o]),
+                                         0.5f * (a.d50[o - 1] + a.d50[o]));
+          float si = a.qs_int_x[o], sc = a.qs_comp_x[o];
+          kahan_add(si, sc, qs * hdt);
+          a.qs_int_x[o] = si;
+          a.qs_comp_x[o] = sc;
+        }
+        if (r >= 1) {
+          const float hf = fmx(s_e[er - 1][ec], s_e[er][ec]) - fmx(s_z[er - 1][ec], s_z[er][ec]);
+          const float qs = face_capacity(s_fyh[lr + 1][lc + 1], hf, hf, 0.5f * (a.n[o - P] + a.n[o]),
+                                         0.5f * (a.d50[o - P] + a.d50[o]));
+          float si = a.qs_int_y[o], sc = a.qs_comp_y[o];
+          kahan_add(si, sc, qs * hdt);
+          a.qs_int_y[o] = si;
+          a.qs_comp_y[o] = sc;
+        }
+      }
       const float dh = -((s_fxh[lr + 1][lc + 2] - s_fxh[lr + 1][lc + 1]) +
                          (s_fyh[lr + 2][lc + 1] - s_fyh[lr + 1][lc + 1])) * inv_dx;
       const float hg = s_h[er][ec];
@@ -377,7 +406,6 @@ __global__ __launch_bounds__(kThreads) void hllc_stage_kernel(HlArgs a) {
       const float src_y = -g * 0.5f * (hp_y + hm_y) * sz_y;
       const float dhv = (-((s_fyl[lr + 2][lc + 1] - s_fyr[lr + 1][lc + 1]) +
                            (s_fxt[lr + 1][lc + 2] - s_fxt[lr + 1][lc + 1])) + src_y) * inv_dx;
-      const int64_t o = (int64_t)r * P + c;
       if (STAGE == 1) {  // _rk_stage1
         const float hh = fmx(hg + dt * dh, 0.0f);
         float hu1 = 0.0f, hv1 = 0.0f;
@@ -504,6 +532,11 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   a.h_comp = s->h_comp;
   a.rain = s->rain;
   a.infil = s->infil;
+  a.d50 = s->d50;
+  a.qs_int_x = s->qs_int_x;
+  a.qs_comp_x = s->qs_comp_x;
+  a.qs_int_y = s->qs_int_y;
+  a.qs_comp_y = s->qs_comp_y;
   a.ctl = s->ctl;
   a.part = s->part;
   a.dx64 = s->dx;
@@ -518,6 +551,7 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   if (s->armed & RBS_ARM_COMP) f |= HL_COMP;
   if (s->armed & RBS_ARM_RAIN) f |= HL_FIELD;
   if (s->armed & RBS_ARM_INFIL) f |= HL_INFIL;
+  if (s->armed & RBS_ARM_SED) f |= HL_SED;
   for (int e = 0; e < 4; ++e)
     if (s->bc[e] != RBS_BC_CLOSED) f |= HL_LOSS;
   a.flags = f;
