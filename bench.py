@@ -152,10 +152,12 @@ def main() -> int:
     st, d = build_state(args.n, args.seed)
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
+        variant = args.variant
     else:
         from rbs.autotune import tune
 
-        log(f"[bench] autotuned LI variant {tune(st)}")
+        variant = tune(st)
+        log(f"[bench] autotuned LI variant {variant}")
     cells = args.n * args.n
     rain_sum = float(d["rain"].astype(np.float64).sum())
     end_time = 10800.0
@@ -198,10 +200,13 @@ def main() -> int:
     sweep = None
     if args.sweep:
         sweep = variant_sweep(st, dt_k, args.kernel_reps, cells, args.variant)
+    # HBM bytes per launch of the variant that ran, from the committed FETCH_SIZE /
+    # WRITE_SIZE summary (null for a variant or grid it does not cover)
     traffic = None
     tf = ROOT / "profiles" / "li_rain_4096_traffic.json"
     if tf.exists() and args.n == 4096:
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        entry = json.loads(tf.read_text()).get("variants", {}).get(str(variant))
+        traffic = entry.get("bytes_per_launch") if entry else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -272,7 +277,7 @@ def variant_sweep(st, dt: float, reps: int, cells: int, keep: int) -> list:
         _lib.call("rbs_time_li_kernel", st._handle, reps, dt, 0.0, 1.0, ctypes.byref(avg))
         us = avg.value * 1e3
         gbs = BYTES_PER_CELL * cells / (us * 1e-6) / 1e9
-        fam_name = ("lds_tile", "strip", "strip2")[fam.value]
+        fam_name = ("lds_tile", "strip", "strip2", "rows")[fam.value]
         rows.append({"variant": v, "family": fam_name, "tile": f"{bx.value}x{by.value}",
                      "default": bool(dflt.value), "kernel_us": round(us, 2),
                      "frac": round(gbs / HBM_PEAK_GBS, 4)})

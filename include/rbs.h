@@ -308,8 +308,9 @@ int rbs_time_li_kernel(rbs_state *st, int32_t reps, double dt, double rain, doub
 int rbs_set_li_variant(rbs_state *st, int32_t variant);
 int rbs_li_variant_count(int32_t *count);
 /* Shape of a variant: the block's tile of cells (bx x by) and its family
- * (0 = LDS tile, 1 = register strips with a column per lane, 2 = register strips
- * with two columns per lane); *is_default = 1 for the library default. */
+ * (0 = LDS tile, 1 = register strips with a column per lane, 3 = register strips side
+ * by side in row blocks whose stores are whole 128-B lines; 2 is not used); *is_default
+ * = 1 for the library default. */
 int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *family,
                         int32_t *is_default);
 /* Tile variants of the LI step of a state armed with sub-grid channels (the LDS

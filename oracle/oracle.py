@@ -97,10 +97,12 @@ def lib() -> ctypes.CDLL:
             getattr(L, fn).restype = ctypes.c_float
         L.orc_manning_denominator_radius.argtypes = [ctypes.c_float] * 6
         L.orc_manning_denominator_radius.restype = ctypes.c_float
-        for fn in ("orc_pow73", "orc_pow43", "orc_cbrt", "orc_pow15"):
+        for fn in ("orc_pow73", "orc_rpow73", "orc_pow43", "orc_cbrt", "orc_pow15"):
             getattr(L, fn).argtypes = [ctypes.c_float]
             getattr(L, fn).restype = ctypes.c_float
         L.orc_set_pow_exact.argtypes = [ctypes.c_int]
+        L.orc_pow_sweep.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_pow_sweep.restype = ctypes.c_long
         L.orc_fill_span.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_fill_span.restype = ctypes.c_double
         _lib = L
@@ -337,6 +339,15 @@ def inject_point_sources_c(st: State, rows, cols, add_depth, comp):
 
 def pow73(x: float) -> float:
     return float(lib().orc_pow73(x))
+
+
+def rpow73(x: float) -> float:
+    return float(lib().orc_rpow73(x))
+
+
+def pow_sweep(kind: int, lo: int, hi: int) -> int:
+    """Float32 bit patterns in [lo, hi) whose power differs from powl's, rounded."""
+    return int(lib().orc_pow_sweep(kind, lo, hi))
 
 
 def pow43(x: float) -> float:

@@ -47,29 +47,31 @@ static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo),
 /*
  * Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
  * Warp's wp.pow is a libm powf, accurate to about an ulp, whose last bit differs
- * between math libraries.  Both this oracle and the HIP kernels evaluate the four
- * powers the solver needs by the same sequence of IEEE basic operations, so the two
- * agree bit for bit, and the result is the correctly rounded float32 power except in
- * vanishingly rare ties: x^(-1/3) from a bit-pattern seed refined by three float32
- * and one float64 Newton step, then the power formed in float64 and rounded once.
- * tests/test_oracle_pow.py checks all four against the exact power.
+ * between math libraries.  Both this oracle and the HIP kernels evaluate the powers
+ * the solver needs by the same sequence of IEEE basic operations (fused multiply-adds
+ * only where written as fmaf/fma, which are correctly rounded), so the two agree bit
+ * for bit, and the result is the correctly rounded float32 power except in vanishingly
+ * rare ties: x^(-1/3) from a bit-pattern seed refined by two float32 Newton steps
+ * (relative error below 2e-5), then one float64 step of third order,
+ * b (1 + r/3 + 2r^2/9 + 14r^3/81) with r = 1 - x b^3 formed exactly by an fma, and
+ * the power formed in float64 and rounded once.  tests/test_oracle_pow.py checks each
+ * against the exact power, every float32 of three binades and random ones elsewhere.
  */
-static inline void inv_cbrt64(float x, double *xd, double *yd) {
+static inline double inv_cbrt64(float x) {
   uint32_t i;
   memcpy(&i, &x, 4);
   uint32_t j = 0x54a2fa8cu - i / 3u;
   float y;
   memcpy(&y, &j, 4);
   const float third = (float)(1.0 / 3.0);
-  for (int k = 0; k < 3; ++k) {
-    float t = ((x * y) * y) * y;
-    y = (y * (4.0f - t)) * third;
+  for (int k = 0; k < 2; ++k) {
+    float r = fmaf(-(x * y), y * y, 1.0f);
+    y = fmaf(y * r, third, y);
   }
   double a = (double)x, b = (double)y;
-  double t = ((a * b) * b) * b;
-  b = (b * (4.0 - t)) * (1.0 / 3.0);
-  *xd = a;
-  *yd = b;
+  double r = fma(-(a * b), b * b, 1.0); /* a*b and b*b are exact */
+  double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
+  return fma(b * r, c, b);
 }
 /* 1: the correctly rounded powers above (the GPU's); 0: the C library's powf, which
  * is what the reference's Warp-CPU runs evaluated (its published round-off-level
@@ -78,20 +80,21 @@ static int g_pow_exact = 1;
 void orc_set_pow_exact(int on) { g_pow_exact = on != 0; }
 float orc_pow73(float x) { /* x^(7/3) = x^3 * x^(-2/3) */
   if (!g_pow_exact) return powf(x, (float)(7.0 / 3.0));
-  double a, b;
-  inv_cbrt64(x, &a, &b);
+  double a = (double)x, b = inv_cbrt64(x);
   return (float)(((a * a) * a) * (b * b));
+}
+float orc_rpow73(float x) { /* x^(-7/3) = (x^(-1/3))^7 */
+  double b = inv_cbrt64(x), b2 = b * b;
+  return (float)(((b2 * b2) * b2) * b);
 }
 float orc_cbrt(float x) { /* x^(1/3) = x * x^(-2/3) */
   if (!g_pow_exact) return powf(x, (float)(1.0 / 3.0));
-  double a, b;
-  inv_cbrt64(x, &a, &b);
+  double a = (double)x, b = inv_cbrt64(x);
   return (float)(a * (b * b));
 }
 float orc_pow43(float x) { /* x^(4/3) = x^2 * x^(-2/3) */
   if (!g_pow_exact) return powf(x, (float)(4.0 / 3.0));
-  double a, b;
-  inv_cbrt64(x, &a, &b);
+  double a = (double)x, b = inv_cbrt64(x);
   return (float)((a * a) * (b * b));
 }
 float orc_pow15(float x) { /* x^(3/2) = x * sqrt(x) */
@@ -100,10 +103,33 @@ float orc_pow15(float x) { /* x^(3/2) = x * sqrt(x) */
   return (float)(a * sqrt(a));
 }
 
-/* friction.py:104-117 manning_denominator */
+/* friction.py:104-117 manning_denominator, 1 + g dt n^2 |q| / h^(7/3).  The exact-power
+ * mode takes the quotient as a product with the correctly rounded h^(-7/3) (one
+ * rounding in place of two, no float32 division), as the HIP kernels do; the libm mode
+ * keeps the reference's division by powf. */
 static inline float manning_denominator(float q_prev, float h_flow, float n, float g,
                                         float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
+  if (!g_pow_exact) return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) * orc_rpow73(h_flow);
+}
+
+/* Every float32 in [lo, hi) (bit patterns) checked against powl rounded to float32;
+ * returns the number that differ.  kind 0..4: x^(7/3), x^(-7/3), x^(1/3), x^(4/3),
+ * x^(3/2).  Test support only. */
+long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi) {
+  static const long double ex[5] = {7.0L / 3.0L, -7.0L / 3.0L, 1.0L / 3.0L, 4.0L / 3.0L, 1.5L};
+  long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (long long u = (long long)lo; u < (long long)hi; ++u) {
+    uint32_t bits = (uint32_t)u;
+    float x;
+    memcpy(&x, &bits, 4);
+    float want = (float)powl((long double)x, ex[kind]);
+    float got = kind == 0 ? orc_pow73(x) : kind == 1 ? orc_rpow73(x) : kind == 2 ? orc_cbrt(x)
+              : kind == 3 ? orc_pow43(x) : orc_pow15(x);
+    bad += got != want;
+  }
+  return bad;
 }
 
 /* friction.py:35-57 manning_denominator_radius: D = 1 + g dt n^2 |q| / (h R^(4/3)) */

@@ -55,14 +55,19 @@ typedef struct orc_state {
   float *qy_ch, *qy_fp;           /* (ny+1, nx) */
 } orc_state;
 
-/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(4/3), x^(1/3), x^(3/2). */
+/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(-7/3), x^(4/3), x^(1/3),
+ * x^(3/2). */
 float orc_pow73(float x);
+float orc_rpow73(float x);
 float orc_pow43(float x);
 float orc_cbrt(float x);
 float orc_pow15(float x);
 /* 1 (default): correctly rounded powers, bit-identical to the HIP kernels;
  * 0: the C library's powf, as the reference's Warp-CPU runs evaluated them. */
 void orc_set_pow_exact(int on);
+/* Test support: how many float32 bit patterns in [lo, hi) give a power (kind 0..4:
+ * x^(7/3), x^(-7/3), x^(1/3), x^(4/3), x^(3/2)) other than powl's, rounded. */
+long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi);
 
 /* Threads used by every pass (OpenMP); 1 gives the serial order of Warp-CPU. */
 void orc_set_threads(int n);

@@ -17,39 +17,44 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 
 // Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
 // The device library's powf and the host's differ in the last bit, and at wet/dry
-// switches that difference grows; so the solver evaluates its four powers by one
-// fixed sequence of IEEE basic operations (no FMA: the build sets
-// -ffp-contract=off), bit-identical on every IEEE machine and correctly rounded but
-// for vanishingly rare ties: x^(-1/3) from a bit-pattern seed, three float32 and one
-// float64 Newton steps, then the power formed in float64 and rounded once.  It is
-// also about a quarter of the instructions of the library powf.
-__device__ __forceinline__ void inv_cbrt64(float x, double &a, double &b) {
+// switches that difference grows; so the solver evaluates its powers by one fixed
+// sequence of IEEE basic operations (the build sets -ffp-contract=off, so the only
+// fused multiply-adds are the fmaf/fma written here, correctly rounded everywhere),
+// bit-identical on every IEEE machine and correctly rounded but for vanishingly rare
+// ties: x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative error
+// below 2e-5), then one float64 step of third order -- b (1 + r/3 + 2r^2/9 + 14r^3/81)
+// with r = 1 - x b^3 formed exactly by an fma -- that leaves it within a few float64
+// ulps; the power is formed in float64 and rounded once.
+__device__ __forceinline__ double inv_cbrt64(float x) {
   const uint32_t j = 0x54a2fa8cu - __float_as_uint(x) / 3u;
   float y = __uint_as_float(j);
   const float third = (float)(1.0 / 3.0);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float t = ((x * y) * y) * y;
-    y = (y * (4.0f - t)) * third;
+  for (int k = 0; k < 2; ++k) {
+    const float r = fmaf(-(x * y), y * y, 1.0f);
+    y = fmaf(y * r, third, y);
   }
-  a = (double)x;
-  b = (double)y;
-  const double t = ((a * b) * b) * b;
-  b = (b * (4.0 - t)) * (1.0 / 3.0);
+  const double a = (double)x, b = (double)y;
+  const double r = fma(-(a * b), b * b, 1.0);  // a*b and b*b are exact
+  // a product and a sum (not an fma) for the r^2 coefficient: no constant pair for
+  // the loop to keep in vector registers
+  const double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
+  return fma(b * r, c, b);
 }
 __device__ __forceinline__ float pow73(float x) {  // x^(7/3) = x^3 * x^(-2/3)
-  double a, b;
-  inv_cbrt64(x, a, b);
+  const double a = (double)x, b = inv_cbrt64(x);
   return (float)(((a * a) * a) * (b * b));
 }
+__device__ __forceinline__ float rpow73(float x) {  // x^(-7/3) = (x^(-1/3))^7
+  const double b = inv_cbrt64(x), b2 = b * b;
+  return (float)(((b2 * b2) * b2) * b);
+}
 __device__ __forceinline__ float cbrt_f(float x) {  // x^(1/3) = x * x^(-2/3)
-  double a, b;
-  inv_cbrt64(x, a, b);
+  const double a = (double)x, b = inv_cbrt64(x);
   return (float)(a * (b * b));
 }
 __device__ __forceinline__ float pow43(float x) {  // x^(4/3) = x^2 * x^(-2/3)
-  double a, b;
-  inv_cbrt64(x, a, b);
+  const double a = (double)x, b = inv_cbrt64(x);
   return (float)((a * a) * (b * b));
 }
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
@@ -57,9 +62,11 @@ __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   return (float)(a * sqrt(a));
 }
 
-// friction.py:104-117 manning_denominator
+// friction.py:104-117 manning_denominator, 1 + g dt n^2 |q| / h^(7/3).  The quotient
+// by the power is taken as a product with the correctly rounded h^(-7/3): one
+// rounding in place of the reference's two, and no float32 division.
 __device__ __forceinline__ float manning_den(float q_prev, float h_flow, float n, float g, float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) / pow73(h_flow);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) * rpow73(h_flow);
 }
 
 // sources.py:274-290 kahan_add

@@ -181,9 +181,10 @@ __device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_
   debt[o] = sc;
 }
 
+template <int NT = kThreads>
 __device__ __forceinline__ void write_partials(const LiArgs &a, int tile, CellSums &sum,
                                                double *s_red) {
-  block_reduce4(sum.m_h, sum.vol, sum.inf, sum.edge, s_red);
+  block_reduce4<NT>(sum.m_h, sum.vol, sum.inf, sum.edge, s_red);
   if (threadIdx.x == 0) {
     a.part.speed_max[tile] = sum.m_h;
     a.part.vol[tile] = sum.vol;
@@ -378,11 +379,31 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t k) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + k * (uint32_t)sizeof(T));
 }
 
-__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
+// The neighbouring lane's value through a DPP wave-wide shift (one VALU move, no LDS
+// crossbar): lane i takes lane i-1's (wave_shr:1) or lane i+1's (wave_shl:1) value,
+// and the lane with no neighbour keeps its own, as __shfl_up / __shfl_down return.
+__device__ __forceinline__ float from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                    0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                    0x130 /* wave_shl:1 */, 0xF, 0xF, false));
+}
 
-template <int H, bool SED>
+// NW > 0 ("row blocks"): the block's NW waves sit side by side on the same H rows,
+// NW * 60 columns whose byte span is a multiple of 128 B, and each finished row's
+// outputs go through LDS so that the block stores them as whole 256-B lines.  A
+// wave's own 60 columns start 8 B off a line, and their direct stores leave the two
+// lines at each strip edge written in parts by two waves at different times, which
+// costs the HBM far more than the bytes (the LI step's store pattern of 60-column
+// strips takes about 11% longer than the same bytes stored in whole lines).  One barrier per row keeps the
+// double-buffered LDS rows in step.
+template <int H, bool SED, int NW = 0>
 __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
+  constexpr bool ROWS = NW > 0;
+  constexpr int BWC = ROWS ? kStripW * NW : 1;  // a row block's own columns
+  static_assert(!ROWS || (BWC * 4) % 128 == 0, "row blocks must span whole 128-B lines");
   const DevCtl *ctl = a.ctl;
   if (ctl->active != 1) return;
   const int cur = ctl->cur;
@@ -394,7 +415,10 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   float *__restrict__ qy_out = a.qy[cur ^ 1];
   const float *__restrict__ zf = a.z;
   const float *__restrict__ nf = a.n;
-  __shared__ double s_red[4 * (kThreads / 64)];
+  constexpr int NT = ROWS ? 64 * NW : kThreads;
+  __shared__ double s_red[4 * (NT / 64)];
+  // row blocks: the finished row's h, qx, qy and Kahan compensation, double-buffered
+  __shared__ float s_row[ROWS ? 2 : 1][4][BWC];
 
   const int ny = a.ny, nx = a.nx;
   const int64_t P = a.pitch;
@@ -409,9 +433,13 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-  const int c = tx * kStripW - 2 + lane;
-  const int y0 = (ty * kStripWaves + wave) * H;
+  const int strip = ROWS ? tx * NW + wave : tx;
+  const int c = strip * kStripW - 2 + lane;
+  const int y0 = ROWS ? ty * H : (ty * kStripWaves + wave) * H;  // block-uniform for ROWS
   const int y_end = y0 + H < ny ? y0 + H : ny;
+  // a row block's waves past the east edge keep only the block's barriers and stores
+  const bool idle = ROWS && strip * kStripW >= nx;
+  const int x0 = ROWS ? tx * BWC : 0;  // the row block's first column
   // clamped reads stay in bounds; unsigned offsets let a load pair a scalar row
   // base with a 32-bit lane offset
   const uint32_t cc = clampi(c, 0, nx - 1);
@@ -421,8 +449,38 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const bool xface_ok = c >= 1 && c <= nx - 1;  // interior x-face left of column c
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
 
+  // row blocks: after the row's barrier, wave w stores the block's columns
+  // [64w, 64w + 64) (and [64(w + NW), ...) when the block spans more lines than waves)
+  auto flush_row = [&](int it, int i) {
+    // every wave of the block reaches this barrier at the same iterations (the rows
+    // are the block's); the other buffer is free again because its readers passed
+    // the previous row's barrier
+    __syncthreads();
+#pragma unroll
+    for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
+      const int j = 64 * (part + wave) + lane;
+      if (j < BWC && x0 + j < nx) {
+        const int64_t row = (int64_t)i * P;  // uniform
+        // an empty asm makes the column opaque, so the loop-invariant per-lane
+        // pointers (four 64-bit values) are not hoisted into registers: each store
+        // pairs the row's scalar base with this 32-bit offset
+        uint32_t col = (uint32_t)(x0 + j);
+        asm volatile("" : "+v"(col));
+        at(h_out + row, col) = s_row[it & 1][0][j];
+        at(qx_out + row, col) = s_row[it & 1][1][j];
+        at(qy_out + row, col) = s_row[it & 1][2][j];
+        if (flags & LI_COMP) at(a.h_comp + row, col) = s_row[it & 1][3][j];
+      }
+    }
+  };
+
   CellSums sum;
-  if (y0 < ny) {  // wave-uniform
+  if (ROWS && idle && y0 < ny) {
+    for (int it = 1; it < H + 4; ++it) {
+      const int i = y0 - 4 + it;
+      if (i >= y0 && i < y_end) flush_row(it, i);
+    }
+  } else if (y0 < ny) {  // wave-uniform (block-uniform for row blocks)
     // the window: row R-1 inputs, updated faces of rows R-1 and R-2, old depths of
     // rows R-1 and R-2, donor factors of rows R-2 and R-3
     float e1 = 0.0f, z1 = 0.0f, n1 = 0.0f;
@@ -486,7 +544,8 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         z2l = from_left(z2);
         n2l = from_left(n2);
       }
-      if (i >= y0 && i < y_end && own) {
+      const bool row_live = i >= y0 && i < y_end;  // block-uniform for ROWS
+      if (row_live && own) {
         float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
         if (limit) limit_faces(qxL, qxR, qyT, qyB, b2, b2l, b2r, b3, b1);
         const int64_t row = (int64_t)i * P;  // uniform
@@ -501,13 +560,22 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         }
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
-        if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
-        at(h_out + row, cu) = h;
-        at(qx_out + row, cu) = qxL;
+        if (ROWS) {
+          const int j = kStripW * wave + lane - 2;  // in [0, BWC)
+          s_row[it & 1][0][j] = h;
+          s_row[it & 1][1][j] = qxL;
+          s_row[it & 1][2][j] = qyT;
+          s_row[it & 1][3][j] = comp;
+        } else {
+          if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
+          at(h_out + row, cu) = h;
+          at(qx_out + row, cu) = qxL;
+          at(qy_out + row, cu) = qyT;
+        }
         if (c == nx - 1) at(qx_out + row, cu + 1) = qxR;
-        at(qy_out + row, cu) = qyT;
         if (i == ny - 1) at(qy_out + row + P, cu) = qyB;
       }
+      if (ROWS && row_live) flush_row(it, i);
       // slide the window
       if (SED) {
         e3 = e2; z3 = z2; n3 = n2;
@@ -520,7 +588,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
       b3 = b2; b2 = b1;
     }
   }
-  write_partials(a, tile, sum, s_red);
+  write_partials<NT>(a, tile, sum, s_red);
 }
 
 // ---- family 3: LDS tile of a state with sub-grid channels --------------------------
@@ -746,10 +814,10 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
 }
 // W: the waves per SIMD the register allocator aims for (the bedload form keeps
 // its extra window registers at 4)
-template <int H, bool SED, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 4 : W)))
-void li_strip_kernel(LiArgs a) {
-  li_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
+template <int H, bool SED, int W, int NW = 0>
+__global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
+__attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_kernel(LiArgs a) {
+  li_strip<H, SED, NW>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
 // A batch of states in one launch per tile segment (seg 0: members without the
@@ -778,13 +846,14 @@ __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__rest
   int m, tile;
   if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) li_tile<BX, BY>(members[m], tile);
 }
-template <int H, bool SED, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 4 : W)))
-void li_strip_batch_kernel(
+template <int H, bool SED, int W, int NW = 0>
+__global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
+__attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_batch_kernel(
     const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
     const int *__restrict__ tile_first, const BatchDims *dims, int seg) {
   int m, tile;
-  if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) li_strip<H, SED>(members[m], tile);
+  if (batch_tile(tile_member, tile_first, dims, seg, m, tile))
+    li_strip<H, SED, NW>(members[m], tile);
 }
 
 // ---- variants (the autotuner picks one per grid shape) -------------------------------
@@ -794,20 +863,27 @@ using StepFn = void (*)(LiArgs);
 using BatchFn = void (*)(const LiArgs *, const int *, const int *, const BatchDims *, int);
 struct LiVariant {
   int bx, by;
-  int family;  // 0 LDS tile, 1 register strip
+  int family;  // 0 LDS tile, 1 register strips, 3 register strips in row blocks
+               // with line-aligned stores (2: retired form, two columns per lane)
   StepFn kern, kern_sed;
   BatchFn batch, batch_sed;
+  int threads;  // block size
 };
 #define RBS_LI_TILE(X, Y)                                                                 \
   {X, Y, 0, li_step_kernel<X, Y>, li_step_kernel<X, Y>, li_batch_kernel<X, Y>,            \
-   li_batch_kernel<X, Y>}
-#define RBS_LI_STRIP(H, W)                                                                    \
+   li_batch_kernel<X, Y>, kThreads}
+#define RBS_LI_STRIP(H, W)                                                                \
   {kStripW, kStripWaves * (H), 1, li_strip_kernel<H, false, W>, li_strip_kernel<H, true, W>,  \
-   li_strip_batch_kernel<H, false, W>, li_strip_batch_kernel<H, true, W>}
+   li_strip_batch_kernel<H, false, W>, li_strip_batch_kernel<H, true, W>, kThreads}
+#define RBS_LI_ROWS(H, W, NW)                                                             \
+  {kStripW * (NW), H, 3, li_strip_kernel<H, false, W, NW>,                                \
+   li_strip_kernel<H, true, W, NW>, li_strip_batch_kernel<H, false, W, NW>,               \
+   li_strip_batch_kernel<H, true, W, NW>, 64 * (NW)}
 static const LiVariant kLiVariants[] = {
-    RBS_LI_TILE(64, 16),  RBS_LI_TILE(64, 8),   RBS_LI_TILE(64, 32),
-    RBS_LI_TILE(128, 8),  RBS_LI_TILE(128, 16), RBS_LI_TILE(32, 32),
-    RBS_LI_STRIP(16, 6),  RBS_LI_STRIP(32, 6),  RBS_LI_STRIP(64, 6),
+    RBS_LI_TILE(64, 16),     RBS_LI_TILE(64, 8),      RBS_LI_TILE(64, 32),
+    RBS_LI_TILE(128, 8),     RBS_LI_TILE(128, 16),    RBS_LI_TILE(32, 32),
+    RBS_LI_STRIP(16, 6),     RBS_LI_STRIP(32, 6),     RBS_LI_STRIP(64, 6),
+    RBS_LI_ROWS(16, 6, 8),   RBS_LI_ROWS(32, 6, 8),   RBS_LI_ROWS(50, 6, 8),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
@@ -1060,7 +1136,7 @@ size_t li_args_size() { return sizeof(LiArgs); }
 int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
                     const int *tile_first, const BatchDims *dims, int grid, hipStream_t stream) {
   const LiVariant &v = variant_at(variant);
-  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(grid), dim3(kThreads), 0, stream,
+  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(grid), dim3(v.threads), 0, stream,
                      members, tile_member, tile_first, dims, sed ? 1 : 0);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
@@ -1082,11 +1158,13 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
     return RBS_E_STATE;
   }
   StepFn fn = (a.flags & LI_SED) ? v.kern_sed : v.kern;
+  int threads = v.threads;
   if (a.flags & LI_CHAN) {
     const ChanVariant &cv = chan_variant_at(s->chan_variant);
     fn = (a.flags & LI_SED) ? cv.kern_sed : cv.kern;
+    threads = kThreads;
   }
-  hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(kThreads), 0, s->stream, a);
+  hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(threads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   s->cur ^= 1;
   return RBS_OK;

@@ -1,9 +1,10 @@
 """LI kernel-variant autotuner.
 
-librbs carries several fused local-inertial kernels (LDS tiles of several shapes and
-register strips of several heights); all of them write the same bits, and which is
-fastest depends on the grid shape (a small grid needs small work units to fill 256
-CUs) and on which sources are armed.  ``best_li_variant`` times every variant on a
+librbs carries several fused local-inertial kernels (LDS tiles of several shapes,
+register strips of several heights, and strips side by side in row blocks whose stores
+are whole cache lines); all of them write the same bits, and which is fastest depends
+on the grid shape (a small grid needs small work units to fill 256 CUs, a wide one
+gains from line-aligned stores) and on which sources are armed.  ``best_li_variant`` times every variant on a
 synthetic state of the given shape and arming (HIP events around each fused kernel,
 ``rbs_time_li_kernel``), keeps the fastest, and caches the choice per key for the
 life of the process.  ``tune`` applies it to a state.
@@ -29,7 +30,7 @@ def variant_table() -> list[dict]:
         bx, by, fam, dflt = (ctypes.c_int32() for _ in range(4))
         _lib.call("rbs_li_variant_info", v, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(fam),
                   ctypes.byref(dflt))
-        out.append({"variant": v, "family": ("lds_tile", "strip", "strip2")[fam.value],
+        out.append({"variant": v, "family": ("lds_tile", "strip", "strip2", "rows")[fam.value],
                     "tile": (bx.value, by.value), "default": bool(dflt.value)})
     return out
 
