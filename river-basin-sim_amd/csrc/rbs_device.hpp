@@ -15,6 +15,17 @@ __device__ __forceinline__ float fclamp(float x, float lo, float hi) { return fm
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Pointers in the global address space.  A kernel argument's pointers are known to be
+// global; pointers read from memory (a batch's per-member argument blocks) are
+// generic, and their accesses compile to flat instructions on 64-bit per-lane
+// addresses.  Cast once, every access is a global load or store with a scalar base.
+template <typename T>
+using gptr = T __attribute__((address_space(1))) *;
+template <typename T>
+__device__ __forceinline__ gptr<T> glob(T *p) {
+  return (gptr<T>)p;
+}
+
 // Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
 // The device library's powf and the host's differ in the last bit, and at wet/dry
 // switches that difference grows; so the solver evaluates its powers by one fixed

@@ -105,7 +105,8 @@ struct StepScalars {
   float dt, dtdx, dxdt, rain, rain_scale, rain_cont;
 };
 
-__device__ __forceinline__ StepScalars step_scalars(const LiArgs &a, const DevCtl *ctl) {
+template <typename CtlPtr>
+__device__ __forceinline__ StepScalars step_scalars(const LiArgs &a, CtlPtr ctl) {
   StepScalars k;
   const double dt64 = ctl->dt;
   k.dt = (float)dt64;
@@ -169,9 +170,10 @@ __device__ __forceinline__ float cell_update(int flags, int r, int c, int ny, in
 
 // accumulate_qs_x / accumulate_qs_y (sediment.py:218-273) for one face: the MPM
 // capacity at the face's flow depth, Kahan-added (times dt) to the integral at `o`.
+template <typename FloatPtr>
 __device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_b, float z_a,
                                                 float z_b, float n_a, float n_b, float d_a,
-                                                float d_b, float dt, float *integ, float *debt,
+                                                float d_b, float dt, FloatPtr integ, FloatPtr debt,
                                                 int64_t o) {
   const float h_face = fmx(eta_a, eta_b) - fmx(z_a, z_b);
   const float qs = face_capacity(q, h_face, h_face, 0.5f * (n_a + n_b), 0.5f * (d_a + d_b));
@@ -186,10 +188,10 @@ __device__ __forceinline__ void write_partials(const LiArgs &a, int tile, CellSu
                                                double *s_red) {
   block_reduce4<NT>(sum.m_h, sum.vol, sum.inf, sum.edge, s_red);
   if (threadIdx.x == 0) {
-    a.part.speed_max[tile] = sum.m_h;
-    a.part.vol[tile] = sum.vol;
-    a.part.infil[tile] = sum.inf;
-    a.part.edge[tile] = sum.edge;
+    glob(a.part.speed_max)[tile] = sum.m_h;
+    glob(a.part.vol)[tile] = sum.vol;
+    glob(a.part.infil)[tile] = sum.inf;
+    glob(a.part.edge)[tile] = sum.edge;
   }
 }
 
@@ -378,6 +380,31 @@ template <typename T>
 __device__ __forceinline__ const T &at(const T *base, uint32_t k) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + k * (uint32_t)sizeof(T));
 }
+template <typename T>
+__device__ __forceinline__ T __attribute__((address_space(1))) &at(gptr<T> base, uint32_t k) {
+  using B = gptr<unsigned char>;
+  return *(gptr<T>)((B)base + k * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T __attribute__((address_space(1))) &at(gptr<const T> base,
+                                                                        uint32_t k) {
+  using B = gptr<const unsigned char>;
+  return *(gptr<const T>)((B)base + k * (uint32_t)sizeof(T));
+}
+// Streaming (non-temporal) access, for the row blocks' whole-line stores and their
+// per-cell source reads, which no neighbouring wave reads again: 1-4 % off the row
+// blocks at 4096^2.  Loads of the rows' inputs stay ordinary -- the neighbouring
+// strips' halo columns and the next block's halo rows hit in L2 -- and so do the
+// strips' own stores, which share their edge lines with the next strip (streaming
+// either of those made the kernels 5-30 % slower).  The values are the same.
+template <typename R>
+__device__ __forceinline__ float ld_nt(const R &ref) {
+  return __builtin_nontemporal_load(&ref);
+}
+template <typename R>
+__device__ __forceinline__ void st_nt(R &ref, float v) {
+  __builtin_nontemporal_store(v, &ref);
+}
 
 // The neighbouring lane's value through a DPP wave-wide shift (one VALU move, no LDS
 // crossbar): lane i takes lane i-1's (wave_shr:1) or lane i+1's (wave_shl:1) value,
@@ -404,17 +431,21 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   constexpr bool ROWS = NW > 0;
   constexpr int BWC = ROWS ? kStripW * NW : 1;  // a row block's own columns
   static_assert(!ROWS || (BWC * 4) % 128 == 0, "row blocks must span whole 128-B lines");
-  const DevCtl *ctl = a.ctl;
+  // every pointer as a global one (a batch reads them from memory)
+  const gptr<const DevCtl> ctl = glob(a.ctl);
   if (ctl->active != 1) return;
   const int cur = ctl->cur;
-  const float *__restrict__ h_in = a.h[cur];
-  float *__restrict__ h_out = a.h[cur ^ 1];
-  const float *__restrict__ qx_in = a.qx[cur];
-  float *__restrict__ qx_out = a.qx[cur ^ 1];
-  const float *__restrict__ qy_in = a.qy[cur];
-  float *__restrict__ qy_out = a.qy[cur ^ 1];
-  const float *__restrict__ zf = a.z;
-  const float *__restrict__ nf = a.n;
+  const gptr<const float> __restrict__ h_in = glob(a.h[cur]);
+  const gptr<float> __restrict__ h_out = glob(a.h[cur ^ 1]);
+  const gptr<const float> __restrict__ qx_in = glob(a.qx[cur]);
+  const gptr<float> __restrict__ qx_out = glob(a.qx[cur ^ 1]);
+  const gptr<const float> __restrict__ qy_in = glob(a.qy[cur]);
+  const gptr<float> __restrict__ qy_out = glob(a.qy[cur ^ 1]);
+  const gptr<const float> __restrict__ zf = glob(a.z);
+  const gptr<const float> __restrict__ nf = glob(a.n);
+  const gptr<float> __restrict__ hcomp = glob(a.h_comp);
+  const gptr<const float> __restrict__ rainf = glob(a.rain);
+  const gptr<const float> __restrict__ infilf = glob(a.infil);
   constexpr int NT = ROWS ? 64 * NW : kThreads;
   __shared__ double s_red[4 * (NT / 64)];
   // row blocks: the finished row's h, qx, qy and Kahan compensation, double-buffered
@@ -466,10 +497,10 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         // pairs the row's scalar base with this 32-bit offset
         uint32_t col = (uint32_t)(x0 + j);
         asm volatile("" : "+v"(col));
-        at(h_out + row, col) = s_row[it & 1][0][j];
-        at(qx_out + row, col) = s_row[it & 1][1][j];
-        at(qy_out + row, col) = s_row[it & 1][2][j];
-        if (flags & LI_COMP) at(a.h_comp + row, col) = s_row[it & 1][3][j];
+        st_nt(at(h_out + row, col), s_row[it & 1][0][j]);
+        st_nt(at(qx_out + row, col), s_row[it & 1][1][j]);
+        st_nt(at(qy_out + row, col), s_row[it & 1][2][j]);
+        if (flags & LI_COMP) st_nt(at(hcomp + row, col), s_row[it & 1][3][j]);
       }
     }
   };
@@ -501,9 +532,9 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     auto sources = [&](int i) {
       if (i >= y0 && i < y_end && own) {
         const int64_t o = (int64_t)i * P;  // uniform
-        if (flags & LI_COMP) pcomp = at(a.h_comp + o, cu);
-        if (flags & LI_FIELD) prain = at(a.rain + o, cu);
-        if (flags & LI_INFIL) pinfil = at(a.infil + o, cu);
+        if (flags & LI_COMP) pcomp = ROWS ? ld_nt(at(hcomp + o, cu)) : at(hcomp + o, cu);
+        if (flags & LI_FIELD) prain = ROWS ? ld_nt(at(rainf + o, cu)) : at(rainf + o, cu);
+        if (flags & LI_INFIL) pinfil = ROWS ? ld_nt(at(infilf + o, cu)) : at(infilf + o, cu);
       }
     };
     // row y0-2 only feeds the y-faces above row y0-1: take its inputs, no faces
@@ -552,11 +583,11 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         const int64_t o = row + c;
         if (SED && (flags & LI_SED)) {
           if (c >= 1)
-            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, a.d50[o - 1], a.d50[o], k.dt,
-                            a.qs_int_x, a.qs_comp_x, o);
+            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, glob(a.d50)[o - 1], glob(a.d50)[o],
+                            k.dt, glob(a.qs_int_x), glob(a.qs_comp_x), o);
           if (i >= 1)
-            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, a.d50[o - P], a.d50[o], k.dt,
-                            a.qs_int_y, a.qs_comp_y, o);
+            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, glob(a.d50)[o - P], glob(a.d50)[o],
+                            k.dt, glob(a.qs_int_y), glob(a.qs_comp_y), o);
         }
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
@@ -567,7 +598,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
           s_row[it & 1][2][j] = qyT;
           s_row[it & 1][3][j] = comp;
         } else {
-          if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
+          if (flags & LI_COMP) at(hcomp + row, cu) = comp;
           at(h_out + row, cu) = h;
           at(qx_out + row, cu) = qxL;
           at(qy_out + row, cu) = qyT;
@@ -893,8 +924,13 @@ constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
 // storage-curve surface and the section (z, n, w, d, n_ch); a face's two components
 // travel beside its total through the window, and the finished row limits its own
 // faces' components by the same donor factor as their totals.
-template <int H, bool SED>
+// NW > 0: row blocks, as for li_strip -- the block's NW strips side by side, each
+// finished row's eight outputs staged in LDS and stored as whole lines.
+template <int H, bool SED, int NW = 0>
 __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
+  constexpr bool ROWS = NW > 0;
+  constexpr int BWC = ROWS ? kStripW * NW : 1;
+  static_assert(!ROWS || (BWC * 4) % 128 == 0, "row blocks must span whole 128-B lines");
   const DevCtl *ctl = a.ctl;
   if (ctl->active != 1) return;
   const int cur = ctl->cur;
@@ -910,7 +946,10 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   float *__restrict__ qxf_out = a.qx_fp[cur ^ 1];
   float *__restrict__ qyc_out = a.qy_ch[cur ^ 1];
   float *__restrict__ qyf_out = a.qy_fp[cur ^ 1];
-  __shared__ double s_red[4 * (kThreads / 64)];
+  constexpr int NT = ROWS ? 64 * NW : kThreads;
+  __shared__ double s_red[4 * (NT / 64)];
+  // row blocks: h, qx, qx_ch, qx_fp, qy, qy_ch, qy_fp, Kahan debt; double-buffered
+  __shared__ float s_row[ROWS ? 2 : 1][8][BWC];
 
   const int ny = a.ny, nx = a.nx;
   const int64_t P = a.pitch;
@@ -923,9 +962,12 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-  const int c = tx * kStripW - 2 + lane;
-  const int y0 = (ty * kStripWaves + wave) * H;
+  const int strip = ROWS ? tx * NW + wave : tx;
+  const int c = strip * kStripW - 2 + lane;
+  const int y0 = ROWS ? ty * H : (ty * kStripWaves + wave) * H;  // block-uniform for ROWS
   const int y_end = y0 + H < ny ? y0 + H : ny;
+  const bool idle = ROWS && strip * kStripW >= nx;
+  const int x0 = ROWS ? tx * BWC : 0;
   const uint32_t cc = clampi(c, 0, nx - 1);
   const uint32_t cf = clampi(c, 0, nx);
   const uint32_t cu = (uint32_t)c;  // used only where 0 <= c < nx
@@ -933,9 +975,32 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   const bool xface_ok = c >= 1 && c <= nx - 1;
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
 
+  auto flush_row = [&](int it, int i) {
+    __syncthreads();  // the block's rows: every wave reaches it at the same iterations
+    float *const outs[8] = {h_out, qx_out, qxc_out, qxf_out, qy_out, qyc_out, qyf_out,
+                            a.h_comp};
+#pragma unroll
+    for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
+      const int j = 64 * (part + wave) + lane;
+      if (j < BWC && x0 + j < nx) {
+        const int64_t row = (int64_t)i * P;  // uniform
+        uint32_t col = (uint32_t)(x0 + j);
+        asm volatile("" : "+v"(col));  // keep the offsets 32-bit (no hoisted pointers)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (q < 7 || (flags & LI_COMP)) st_nt(at(outs[q] + row, col), s_row[it & 1][q][j]);
+      }
+    }
+  };
+
   CellSums sum;
   float m_col = 0.0f;  // reduce_column_max starts at 0
-  if (y0 < ny) {  // wave-uniform
+  if (ROWS && idle && y0 < ny) {
+    for (int it = 1; it < H + 4; ++it) {
+      const int i = y0 - 4 + it;
+      if (i >= y0 && i < y_end) flush_row(it, i);
+    }
+  } else if (y0 < ny) {  // wave-uniform (block-uniform for row blocks)
     // window: the cells of rows R-1 (c1), R-2 and R-3 (c2, c3: bedload only; else
     // just c2's width and depth for the column depth), the totals and components of
     // the faces of rows R-1 and R-2, old depths of rows R-1 and R-2, donor factors of
@@ -965,9 +1030,9 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
     auto sources = [&](int i) {
       if (i >= y0 && i < y_end && own) {
         const int64_t o = (int64_t)i * P;
-        if (flags & LI_COMP) pcomp = at(a.h_comp + o, cu);
-        if (flags & LI_FIELD) prain = at(a.rain + o, cu);
-        if (flags & LI_INFIL) pinfil = at(a.infil + o, cu);
+        if (flags & LI_COMP) pcomp = ROWS ? ld_nt(at(a.h_comp + o, cu)) : at(a.h_comp + o, cu);
+        if (flags & LI_FIELD) prain = ROWS ? ld_nt(at(a.rain + o, cu)) : at(a.rain + o, cu);
+        if (flags & LI_INFIL) pinfil = ROWS ? ld_nt(at(a.infil + o, cu)) : at(a.infil + o, cu);
       }
     };
     ingest(y0 - 2);
@@ -1006,7 +1071,8 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
       ChanCell c2l{};
       if (SED) c2l = ChanCell{from_left(c2.eta), from_left(c2.z), from_left(c2.n),
                               from_left(c2.w), from_left(c2.d), from_left(c2.nc)};
-      if (i >= y0 && i < y_end && own) {
+      const bool row_live = i >= y0 && i < y_end;  // block-uniform for ROWS
+      if (row_live && own) {
         float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
         float cxL = fxc2, fxL = fxf2, cyT = fyc2, fyT = fyf2;
         if (limit) {
@@ -1037,25 +1103,33 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
         m_col = fmx(m_col, column_depth_u(h, c2.w, c2.d, dx, rdx));
-        if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
-        at(h_out + row, cu) = h;
-        at(qx_out + row, cu) = qxL;
-        at(qxc_out + row, cu) = cxL;
-        at(qxf_out + row, cu) = fxL;
+        if (ROWS) {
+          const int j = kStripW * wave + lane - 2;  // in [0, BWC)
+          const float v[8] = {h, qxL, cxL, fxL, qyT, cyT, fyT, comp};
+#pragma unroll
+          for (int q = 0; q < 8; ++q) s_row[it & 1][q][j] = v[q];
+        } else {
+          if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
+          at(h_out + row, cu) = h;
+          at(qx_out + row, cu) = qxL;
+          at(qxc_out + row, cu) = cxL;
+          at(qxf_out + row, cu) = fxL;
+          at(qy_out + row, cu) = qyT;
+          at(qyc_out + row, cu) = cyT;
+          at(qyf_out + row, cu) = fyT;
+        }
         if (c == nx - 1) {  // the closed east edge face: total and components 0
           at(qx_out + row, cu + 1) = qxR;
           at(qxc_out + row, cu + 1) = 0.0f;
           at(qxf_out + row, cu + 1) = 0.0f;
         }
-        at(qy_out + row, cu) = qyT;
-        at(qyc_out + row, cu) = cyT;
-        at(qyf_out + row, cu) = fyT;
         if (i == ny - 1) {
           at(qy_out + row + P, cu) = qyB;
           at(qyc_out + row + P, cu) = 0.0f;
           at(qyf_out + row + P, cu) = 0.0f;
         }
       }
+      if (ROWS && row_live) flush_row(it, i);
       // slide the window
       if (SED) c3 = c2;
       c2 = c1;
@@ -1071,7 +1145,7 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
     }
   }
   sum.m_h = m_col;
-  write_partials(a, tile, sum, s_red);
+  write_partials<NT>(a, tile, sum, s_red);
 }
 
 template <int BX, int BY>
@@ -1080,25 +1154,30 @@ __global__ __launch_bounds__(kThreads) void li_chan_kernel(LiArgs a) {
 }
 // W: the waves per SIMD the register allocator aims for; the bedload form keeps its
 // two extra rows of cells at 3 (fewer registers would spill)
-template <int H, bool SED, int W>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SED ? 3 : W)))
-void li_chan_strip_kernel(LiArgs a) {
-  li_chan_strip<H, SED>(a, xcd_tile(blockIdx.x, a.n_tiles));
+template <int H, bool SED, int W, int NW = 0>
+__global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
+__attribute__((amdgpu_waves_per_eu(SED ? 3 : W))) void li_chan_strip_kernel(LiArgs a) {
+  li_chan_strip<H, SED, NW>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 struct ChanVariant {
   int bx, by;
   StepFn kern, kern_sed;
+  int threads;  // block size
 };
-#define RBS_CHAN_TILE(X, Y) {X, Y, li_chan_kernel<X, Y>, li_chan_kernel<X, Y>}
+#define RBS_CHAN_TILE(X, Y) {X, Y, li_chan_kernel<X, Y>, li_chan_kernel<X, Y>, kThreads}
 #define RBS_CHAN_STRIP(H, W)                                                    \
   {kStripW, kStripWaves * (H), li_chan_strip_kernel<H, false, W>,               \
-   li_chan_strip_kernel<H, true, W>}
+   li_chan_strip_kernel<H, true, W>, kThreads}
+#define RBS_CHAN_ROWS(H, W, NW)                                                 \
+  {kStripW * (NW), H, li_chan_strip_kernel<H, false, W, NW>,                    \
+   li_chan_strip_kernel<H, true, W, NW>, 64 * (NW)}
 static const ChanVariant kChanVariants[] = {
     RBS_CHAN_TILE(64, 8), RBS_CHAN_TILE(32, 16), RBS_CHAN_TILE(32, 8), RBS_CHAN_TILE(16, 16),
     RBS_CHAN_STRIP(8, 4), RBS_CHAN_STRIP(16, 4), RBS_CHAN_STRIP(32, 4), RBS_CHAN_STRIP(4, 4),
+    RBS_CHAN_ROWS(8, 4, 8), RBS_CHAN_ROWS(16, 4, 8), RBS_CHAN_ROWS(32, 4, 8),
 };
 constexpr int kNumChanVariants = sizeof(kChanVariants) / sizeof(kChanVariants[0]);
-constexpr int kDefaultChanVariant = 4;  // strips of 8 rows: fastest measured at 4096^2
+constexpr int kDefaultChanVariant = 9;  // row blocks of 16 rows: fastest measured at 4096^2
 static const ChanVariant &chan_variant_at(int v) {
   if (v < 0 || v >= kNumChanVariants) v = kDefaultChanVariant;
   return kChanVariants[v];
@@ -1162,7 +1241,7 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
   if (a.flags & LI_CHAN) {
     const ChanVariant &cv = chan_variant_at(s->chan_variant);
     fn = (a.flags & LI_SED) ? cv.kern_sed : cv.kern;
-    threads = kThreads;
+    threads = cv.threads;
   }
   hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(threads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
