@@ -426,8 +426,12 @@ __device__ __forceinline__ float from_right(float v) {
 // costs the HBM far more than the bytes (the LI step's store pattern of 60-column
 // strips takes about 11% longer than the same bytes stored in whole lines).  One barrier per row keeps the
 // double-buffered LDS rows in step.
-template <int H, bool SED, int NW = 0>
+template <int H, bool SED, int NW = 0, int FL = 1>
 __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
+  // FL: finished rows stored per barrier, from a ring of 2 FL LDS rows (two rows per
+  // barrier measured 3 % faster than one at 4096^2; three no better, four cost
+  // occupancy)
+  static_assert(FL >= 1 && FL <= 4, "one to four rows per barrier");
   constexpr bool ROWS = NW > 0;
   constexpr int BWC = ROWS ? kStripW * NW : 1;  // a row block's own columns
   static_assert(!ROWS || (BWC * 4) % 128 == 0, "row blocks must span whole 128-B lines");
@@ -449,7 +453,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   constexpr int NT = ROWS ? 64 * NW : kThreads;
   __shared__ double s_red[4 * (NT / 64)];
   // row blocks: the finished row's h, qx, qy and Kahan compensation, double-buffered
-  __shared__ float s_row[ROWS ? 2 : 1][4][BWC];
+  __shared__ float s_row[ROWS ? 2 * FL : 1][4][BWC];
 
   const int ny = a.ny, nx = a.nx;
   const int64_t P = a.pitch;
@@ -480,27 +484,35 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const bool xface_ok = c >= 1 && c <= nx - 1;  // interior x-face left of column c
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
 
-  // row blocks: after the row's barrier, wave w stores the block's columns
-  // [64w, 64w + 64) (and [64(w + NW), ...) when the block spans more lines than waves)
-  auto flush_row = [&](int it, int i) {
-    // every wave of the block reaches this barrier at the same iterations (the rows
-    // are the block's); the other buffer is free again because its readers passed
-    // the previous row's barrier
+  // row blocks: the finished row i (iteration it) sits in LDS row slot it % (2 FL);
+  // after every FL-th row of the block (and its last), one barrier, then wave w stores
+  // the pending rows' columns [64w, 64w + 64) (and [64(w + NW), ...) when the block
+  // spans more lines than waves).  Every wave reaches the barriers at the same
+  // iterations (the rows are the block's), and a slot is written again only 2 FL rows
+  // later, after a barrier that its readers had to pass first.
+  auto flush_rows = [&](int it, int i) {
+    const int off = i - y0;
+    if ((off % FL) != FL - 1 && i != y_end - 1) return;  // block-uniform
     __syncthreads();
 #pragma unroll
-    for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
-      const int j = 64 * (part + wave) + lane;
-      if (j < BWC && x0 + j < nx) {
-        const int64_t row = (int64_t)i * P;  // uniform
-        // an empty asm makes the column opaque, so the loop-invariant per-lane
-        // pointers (four 64-bit values) are not hoisted into registers: each store
-        // pairs the row's scalar base with this 32-bit offset
-        uint32_t col = (uint32_t)(x0 + j);
-        asm volatile("" : "+v"(col));
-        st_nt(at(h_out + row, col), s_row[it & 1][0][j]);
-        st_nt(at(qx_out + row, col), s_row[it & 1][1][j]);
-        st_nt(at(qy_out + row, col), s_row[it & 1][2][j]);
-        if (flags & LI_COMP) st_nt(at(hcomp + row, col), s_row[it & 1][3][j]);
+    for (int back = FL - 1; back >= 0; --back) {
+      if (back > off % FL) continue;  // a short last group
+      const int slot = (it - back) % (2 * FL);  // it >= 4 for a live row
+      const int64_t row = (int64_t)(i - back) * P;  // uniform
+#pragma unroll
+      for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
+        const int j = 64 * (part + wave) + lane;
+        if (j < BWC && x0 + j < nx) {
+          // an empty asm makes the column opaque, so the loop-invariant per-lane
+          // pointers (four 64-bit values) are not hoisted into registers: each store
+          // pairs the row's scalar base with this 32-bit offset
+          uint32_t col = (uint32_t)(x0 + j);
+          asm volatile("" : "+v"(col));
+          st_nt(at(h_out + row, col), s_row[slot][0][j]);
+          st_nt(at(qx_out + row, col), s_row[slot][1][j]);
+          st_nt(at(qy_out + row, col), s_row[slot][2][j]);
+          if (flags & LI_COMP) st_nt(at(hcomp + row, col), s_row[slot][3][j]);
+        }
       }
     }
   };
@@ -509,7 +521,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   if (ROWS && idle && y0 < ny) {
     for (int it = 1; it < H + 4; ++it) {
       const int i = y0 - 4 + it;
-      if (i >= y0 && i < y_end) flush_row(it, i);
+      if (i >= y0 && i < y_end) flush_rows(it, i);
     }
   } else if (y0 < ny) {  // wave-uniform (block-uniform for row blocks)
     // the window: row R-1 inputs, updated faces of rows R-1 and R-2, old depths of
@@ -593,10 +605,11 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
                                     rain_v, infil_v, sum);
         if (ROWS) {
           const int j = kStripW * wave + lane - 2;  // in [0, BWC)
-          s_row[it & 1][0][j] = h;
-          s_row[it & 1][1][j] = qxL;
-          s_row[it & 1][2][j] = qyT;
-          s_row[it & 1][3][j] = comp;
+          const int slot = it % (2 * FL);
+          s_row[slot][0][j] = h;
+          s_row[slot][1][j] = qxL;
+          s_row[slot][2][j] = qyT;
+          s_row[slot][3][j] = comp;
         } else {
           if (flags & LI_COMP) at(hcomp + row, cu) = comp;
           at(h_out + row, cu) = h;
@@ -606,7 +619,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         if (c == nx - 1) at(qx_out + row, cu + 1) = qxR;
         if (i == ny - 1) at(qy_out + row + P, cu) = qyB;
       }
-      if (ROWS && row_live) flush_row(it, i);
+      if (ROWS && row_live) flush_rows(it, i);
       // slide the window
       if (SED) {
         e3 = e2; z3 = z2; n3 = n2;
@@ -845,10 +858,10 @@ __global__ __launch_bounds__(kThreads) void li_step_kernel(LiArgs a) {
 }
 // W: the waves per SIMD the register allocator aims for (the bedload form keeps
 // its extra window registers at 4)
-template <int H, bool SED, int W, int NW = 0>
+template <int H, bool SED, int W, int NW = 0, int FL = 1>
 __global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
 __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_kernel(LiArgs a) {
-  li_strip<H, SED, NW>(a, xcd_tile(blockIdx.x, a.n_tiles));
+  li_strip<H, SED, NW, FL>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 
 // A batch of states in one launch per tile segment (seg 0: members without the
@@ -877,14 +890,14 @@ __global__ __launch_bounds__(kThreads) void li_batch_kernel(const LiArgs *__rest
   int m, tile;
   if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) li_tile<BX, BY>(members[m], tile);
 }
-template <int H, bool SED, int W, int NW = 0>
+template <int H, bool SED, int W, int NW = 0, int FL = 1>
 __global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
 __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_batch_kernel(
     const LiArgs *__restrict__ members, const int *__restrict__ tile_member,
     const int *__restrict__ tile_first, const BatchDims *dims, int seg) {
   int m, tile;
   if (batch_tile(tile_member, tile_first, dims, seg, m, tile))
-    li_strip<H, SED, NW>(members[m], tile);
+    li_strip<H, SED, NW, FL>(members[m], tile);
 }
 
 // ---- variants (the autotuner picks one per grid shape) -------------------------------
@@ -906,15 +919,15 @@ struct LiVariant {
 #define RBS_LI_STRIP(H, W)                                                                \
   {kStripW, kStripWaves * (H), 1, li_strip_kernel<H, false, W>, li_strip_kernel<H, true, W>,  \
    li_strip_batch_kernel<H, false, W>, li_strip_batch_kernel<H, true, W>, kThreads}
-#define RBS_LI_ROWS(H, W, NW)                                                             \
-  {kStripW * (NW), H, 3, li_strip_kernel<H, false, W, NW>,                                \
-   li_strip_kernel<H, true, W, NW>, li_strip_batch_kernel<H, false, W, NW>,               \
-   li_strip_batch_kernel<H, true, W, NW>, 64 * (NW)}
+#define RBS_LI_ROWS(H, W, NW, FL)                                                         \
+  {kStripW * (NW), H, 3, li_strip_kernel<H, false, W, NW, FL>,                            \
+   li_strip_kernel<H, true, W, NW, FL>, li_strip_batch_kernel<H, false, W, NW, FL>,       \
+   li_strip_batch_kernel<H, true, W, NW, FL>, 64 * (NW)}
 static const LiVariant kLiVariants[] = {
     RBS_LI_TILE(64, 16),     RBS_LI_TILE(64, 8),      RBS_LI_TILE(64, 32),
     RBS_LI_TILE(128, 8),     RBS_LI_TILE(128, 16),    RBS_LI_TILE(32, 32),
     RBS_LI_STRIP(16, 6),     RBS_LI_STRIP(32, 6),     RBS_LI_STRIP(64, 6),
-    RBS_LI_ROWS(16, 6, 8),   RBS_LI_ROWS(32, 6, 8),   RBS_LI_ROWS(50, 6, 8),
+    RBS_LI_ROWS(16, 6, 8, 2), RBS_LI_ROWS(32, 6, 8, 2), RBS_LI_ROWS(50, 6, 8, 2),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
@@ -926,8 +939,9 @@ constexpr int kDefaultLiVariant = 7;  // strips of 32 rows
 // faces' components by the same donor factor as their totals.
 // NW > 0: row blocks, as for li_strip -- the block's NW strips side by side, each
 // finished row's eight outputs staged in LDS and stored as whole lines.
-template <int H, bool SED, int NW = 0>
+template <int H, bool SED, int NW = 0, int FL = 1>
 __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
+  static_assert(FL >= 1 && FL <= 4, "one to four rows per barrier");
   constexpr bool ROWS = NW > 0;
   constexpr int BWC = ROWS ? kStripW * NW : 1;
   static_assert(!ROWS || (BWC * 4) % 128 == 0, "row blocks must span whole 128-B lines");
@@ -949,7 +963,7 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   constexpr int NT = ROWS ? 64 * NW : kThreads;
   __shared__ double s_red[4 * (NT / 64)];
   // row blocks: h, qx, qx_ch, qx_fp, qy, qy_ch, qy_fp, Kahan debt; double-buffered
-  __shared__ float s_row[ROWS ? 2 : 1][8][BWC];
+  __shared__ float s_row[ROWS ? 2 * FL : 1][8][BWC];
 
   const int ny = a.ny, nx = a.nx;
   const int64_t P = a.pitch;
@@ -975,20 +989,28 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   const bool xface_ok = c >= 1 && c <= nx - 1;
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
 
-  auto flush_row = [&](int it, int i) {
+  // the ring of 2 FL LDS rows and one barrier per FL rows, as in li_strip
+  auto flush_rows = [&](int it, int i) {
+    const int off = i - y0;
+    if ((off % FL) != FL - 1 && i != y_end - 1) return;  // block-uniform
     __syncthreads();  // the block's rows: every wave reaches it at the same iterations
     float *const outs[8] = {h_out, qx_out, qxc_out, qxf_out, qy_out, qyc_out, qyf_out,
                             a.h_comp};
 #pragma unroll
-    for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
-      const int j = 64 * (part + wave) + lane;
-      if (j < BWC && x0 + j < nx) {
-        const int64_t row = (int64_t)i * P;  // uniform
-        uint32_t col = (uint32_t)(x0 + j);
-        asm volatile("" : "+v"(col));  // keep the offsets 32-bit (no hoisted pointers)
+    for (int back = FL - 1; back >= 0; --back) {
+      if (back > off % FL) continue;  // a short last group
+      const int slot = (it - back) % (2 * FL);  // it >= 4 for a live row
+      const int64_t row = (int64_t)(i - back) * P;  // uniform
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (q < 7 || (flags & LI_COMP)) st_nt(at(outs[q] + row, col), s_row[it & 1][q][j]);
+      for (int part = 0; part < (BWC + 63) / 64; part += (NW > 0 ? NW : 1)) {
+        const int j = 64 * (part + wave) + lane;
+        if (j < BWC && x0 + j < nx) {
+          uint32_t col = (uint32_t)(x0 + j);
+          asm volatile("" : "+v"(col));  // keep the offsets 32-bit (no hoisted pointers)
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (q < 7 || (flags & LI_COMP)) st_nt(at(outs[q] + row, col), s_row[slot][q][j]);
+        }
       }
     }
   };
@@ -998,7 +1020,7 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
   if (ROWS && idle && y0 < ny) {
     for (int it = 1; it < H + 4; ++it) {
       const int i = y0 - 4 + it;
-      if (i >= y0 && i < y_end) flush_row(it, i);
+      if (i >= y0 && i < y_end) flush_rows(it, i);
     }
   } else if (y0 < ny) {  // wave-uniform (block-uniform for row blocks)
     // window: the cells of rows R-1 (c1), R-2 and R-3 (c2, c3: bedload only; else
@@ -1107,7 +1129,7 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
           const int j = kStripW * wave + lane - 2;  // in [0, BWC)
           const float v[8] = {h, qxL, cxL, fxL, qyT, cyT, fyT, comp};
 #pragma unroll
-          for (int q = 0; q < 8; ++q) s_row[it & 1][q][j] = v[q];
+          for (int q = 0; q < 8; ++q) s_row[it % (2 * FL)][q][j] = v[q];
         } else {
           if (flags & LI_COMP) at(a.h_comp + row, cu) = comp;
           at(h_out + row, cu) = h;
@@ -1129,7 +1151,7 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
           at(qyf_out + row + P, cu) = 0.0f;
         }
       }
-      if (ROWS && row_live) flush_row(it, i);
+      if (ROWS && row_live) flush_rows(it, i);
       // slide the window
       if (SED) c3 = c2;
       c2 = c1;
@@ -1154,10 +1176,10 @@ __global__ __launch_bounds__(kThreads) void li_chan_kernel(LiArgs a) {
 }
 // W: the waves per SIMD the register allocator aims for; the bedload form keeps its
 // two extra rows of cells at 3 (fewer registers would spill)
-template <int H, bool SED, int W, int NW = 0>
+template <int H, bool SED, int W, int NW = 0, int FL = 1>
 __global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
 __attribute__((amdgpu_waves_per_eu(SED ? 3 : W))) void li_chan_strip_kernel(LiArgs a) {
-  li_chan_strip<H, SED, NW>(a, xcd_tile(blockIdx.x, a.n_tiles));
+  li_chan_strip<H, SED, NW, FL>(a, xcd_tile(blockIdx.x, a.n_tiles));
 }
 struct ChanVariant {
   int bx, by;
@@ -1168,13 +1190,13 @@ struct ChanVariant {
 #define RBS_CHAN_STRIP(H, W)                                                    \
   {kStripW, kStripWaves * (H), li_chan_strip_kernel<H, false, W>,               \
    li_chan_strip_kernel<H, true, W>, kThreads}
-#define RBS_CHAN_ROWS(H, W, NW)                                                 \
-  {kStripW * (NW), H, li_chan_strip_kernel<H, false, W, NW>,                    \
-   li_chan_strip_kernel<H, true, W, NW>, 64 * (NW)}
+#define RBS_CHAN_ROWS(H, W, NW, FL)                                             \
+  {kStripW * (NW), H, li_chan_strip_kernel<H, false, W, NW, FL>,                \
+   li_chan_strip_kernel<H, true, W, NW, FL>, 64 * (NW)}
 static const ChanVariant kChanVariants[] = {
     RBS_CHAN_TILE(64, 8), RBS_CHAN_TILE(32, 16), RBS_CHAN_TILE(32, 8), RBS_CHAN_TILE(16, 16),
     RBS_CHAN_STRIP(8, 4), RBS_CHAN_STRIP(16, 4), RBS_CHAN_STRIP(32, 4), RBS_CHAN_STRIP(4, 4),
-    RBS_CHAN_ROWS(8, 4, 8), RBS_CHAN_ROWS(16, 4, 8), RBS_CHAN_ROWS(32, 4, 8),
+    RBS_CHAN_ROWS(8, 4, 8, 2), RBS_CHAN_ROWS(16, 4, 8, 2), RBS_CHAN_ROWS(32, 4, 8, 2),
 };
 constexpr int kNumChanVariants = sizeof(kChanVariants) / sizeof(kChanVariants[0]);
 constexpr int kDefaultChanVariant = 9;  // row blocks of 16 rows: fastest measured at 4096^2
