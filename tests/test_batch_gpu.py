@@ -54,3 +54,41 @@ def test_batch_members_equal_solo_runs():
         assert np.array_equal(a.state.qy.numpy(), c.state.qy.numpy())
         assert la.rain_in == lc.rain_in and la.edge_out == lc.edge_out
     b.close()
+
+
+def test_row_block_batches_equal_solo_runs():
+    """The row-block kernel (eight strips per block, rows stored through LDS, an idle
+    wave past each member's east edge) in a batch: bitwise its solo runs, ledger too."""
+    import ctypes
+
+    from rbs import _lib
+    from rbs.autotune import variant_table
+    from rbs.ensemble import Batch, ensemble_inputs
+
+    rows = [r["variant"] for r in variant_table() if r["family"] == "rows"]
+    assert rows
+    members = ensemble_inputs(5, seed=11, n_min=100, n_max=700)
+    ends = [600.0, 300.0, 900.0, 600.0, 450.0]
+    for v in rows:
+        solo = [_setup([m], e)[0] for m, e in zip(members, ends)]
+        for run, e in zip(solo, ends):
+            _lib.call("rbs_set_li_variant", run.state._handle, ctypes.c_int32(v))
+            run.run_until(e)
+        batched = [_setup([m], e)[0] for m, e in zip(members, ends)]
+        b = Batch(8)
+        b.set_variants(v, b.variants()[1])
+        slots = [b.add(r, e) for r, e in zip(batched[:3], ends[:3])]
+        b.advance(24)
+        slots += [b.add(r, e) for r, e in zip(batched[3:], ends[3:])]
+        b.run_to_completion(32)
+        for s in slots:
+            b.remove(s)
+        for a, c in zip(solo, batched):
+            la, lc = a.state.ledger(), c.state.ledger()
+            assert la.steps == lc.steps and la.t == lc.t, v
+            assert np.array_equal(a.state.h.numpy(), c.state.h.numpy()), v
+            assert np.array_equal(a.state.qx.numpy(), c.state.qx.numpy()), v
+            assert np.array_equal(a.state.qy.numpy(), c.state.qy.numpy()), v
+            assert la.rain_in == lc.rain_in and la.edge_out == lc.edge_out, v
+            assert la.volume == lc.volume, v
+        b.close()
