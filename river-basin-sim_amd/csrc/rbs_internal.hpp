@@ -225,6 +225,7 @@ int hllc_default_variant();
 void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
+int li_initial_variant(int nx);
 int chan_variant_count();
 int chan_default_variant();
 void chan_variant_shape(int v, int *bx, int *by);

@@ -297,7 +297,7 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
     s->li_variant = v;
     nblk = std::max(nblk, li_num_blocks(s));
   }
-  s->li_variant = li_default_variant();
+  s->li_variant = li_initial_variant(s->nx);
   nblk = std::max(nblk, reduce_partials_needed(s));
   for (int v = 0; v < hllc_variant_count(); ++v) {
     s->hllc_variant = v;
