@@ -309,3 +309,45 @@ def test_device_driven_run_matches_host_loop(rbs):
     assert rd.inflow_cum == pytest.approx(rh.inflow_cum, rel=1e-12)
     assert dev.ledger().t == pytest.approx(600.0, abs=1e-9)
     assert rd.rel_error < 1e-6
+
+
+def test_row_blocks_at_full_width_equal_the_strips(rbs):
+    """The headline kernel family at a width where row blocks end ragged (9 blocks of 480
+    columns over 4000, the last with three idle waves) and a height that is not a
+    multiple of the block: 30 steps with every source armed, bitwise the strip kernel's
+    fields, dt sequence and wave-speed maximum."""
+    import ctypes
+
+    from rbs.autotune import variant_table
+
+    table = variant_table()
+    rows = [r["variant"] for r in table if r["family"] == "rows"]
+    strip = next(r["variant"] for r in table if r["family"] == "strip")
+    rng = np.random.default_rng(21)
+    ny, nx = 1000, 4000
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.0005 * (nx - xx) + 0.001 * yy + 0.5 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    h = np.maximum(1.5 - 0.7 * z, 0.0).astype(np.float32)
+    rain = rng.uniform(0, 3, (ny, nx)).astype(np.float32) * 1e-5
+    infil = rng.uniform(0, 4, (ny, nx)).astype(np.float32) * 1e-6
+    outs = []
+    for v in [strip] + rows:
+        st = rbs.State.from_bed(z, 10.0, depth=h, manning=0.035)
+        st.set_rain_field(rain)
+        st.set_infiltration(infil)
+        st.set_open_boundaries({"east": "open", "south": "open"})
+        st.arm_source_compensation()
+        rbs.lib.call("rbs_set_li_variant", st._handle, ctypes.c_int32(v))
+        dts = []
+        for _ in range(30):
+            dt = rbs.li.compute_dt(st, 0.7, 5.0)
+            dts.append(dt)
+            rbs.li.step(st, dt, rain_scale=1.0)
+        outs.append((v, dts, st.h.numpy(), st.qx.numpy(), st.qy.numpy(), st.h_comp.numpy()))
+        st.close()
+    _, dts0, *f0 = outs[0]
+    for v, dts, *f in outs[1:]:
+        assert dts == dts0, v
+        for name, a, b in zip(("h", "qx", "qy", "h_comp"), f0, f):
+            assert np.array_equal(a, b), f"variant {v}: {name} differs"
+    assert np.abs(outs[0][3]).max() > 0.0
