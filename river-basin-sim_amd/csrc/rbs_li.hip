@@ -391,12 +391,16 @@ __device__ __forceinline__ const T __attribute__((address_space(1))) &at(gptr<co
   using B = gptr<const unsigned char>;
   return *(gptr<const T>)((B)base + k * (uint32_t)sizeof(T));
 }
-// Streaming (non-temporal) access, for the row blocks' whole-line stores and their
-// per-cell source reads, which no neighbouring wave reads again: 1-4 % off the row
-// blocks at 4096^2.  Loads of the rows' inputs stay ordinary -- the neighbouring
-// strips' halo columns and the next block's halo rows hit in L2 -- and so do the
-// strips' own stores, which share their edge lines with the next strip (streaming
-// either of those made the kernels 5-30 % slower).  The values are the same.
+// Streaming (non-temporal) access in the row blocks for the fields a step rewrites --
+// h, qx, qy and the Kahan debt, read once and written once per step -- and ordinary
+// access for the static inputs (bed, Manning n, rain and infiltration fields), which
+// are read again every step and so can stay in the last-level cache between steps
+// while the step's own traffic streams past them (at 4096^2: 179 -> 170-173 us;
+// streaming the static inputs too made it slower).  The channel step's static inputs
+// (seven fields) are more than that cache holds: its row blocks stream their sources
+// and read the rest ordinarily, which measured 3 % faster there.  The strips keep
+// ordinary access: their edge lines are shared with the next strip.  The values are
+// the same.
 template <typename R>
 __device__ __forceinline__ float ld_nt(const R &ref) {
   return __builtin_nontemporal_load(&ref);
@@ -535,18 +539,19 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     float pcomp = 0.0f, prain = 0.0f, pinfil = 0.0f;
     auto ingest = [&](int R) {
       const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;  // uniform
-      ph = at(h_in + o, cc);
+      const int64_t oy = (int64_t)clampi(R, 0, ny) * P;
+      ph = ROWS ? ld_nt(at(h_in + o, cc)) : at(h_in + o, cc);
       pz = at(zf + o, cc);
       pn = at(nf + o, cc);
-      pqx = at(qx_in + o, cf);
-      pqy = at(qy_in + (int64_t)clampi(R, 0, ny) * P, cc);
+      pqx = ROWS ? ld_nt(at(qx_in + o, cf)) : at(qx_in + o, cf);
+      pqy = ROWS ? ld_nt(at(qy_in + oy, cc)) : at(qy_in + oy, cc);
     };
     auto sources = [&](int i) {
       if (i >= y0 && i < y_end && own) {
         const int64_t o = (int64_t)i * P;  // uniform
         if (flags & LI_COMP) pcomp = ROWS ? ld_nt(at(hcomp + o, cu)) : at(hcomp + o, cu);
-        if (flags & LI_FIELD) prain = ROWS ? ld_nt(at(rainf + o, cu)) : at(rainf + o, cu);
-        if (flags & LI_INFIL) pinfil = ROWS ? ld_nt(at(infilf + o, cu)) : at(infilf + o, cu);
+        if (flags & LI_FIELD) prain = at(rainf + o, cu);
+        if (flags & LI_INFIL) pinfil = at(infilf + o, cu);
       }
     };
     // row y0-2 only feeds the y-faces above row y0-1: take its inputs, no faces
