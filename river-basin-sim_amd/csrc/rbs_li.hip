@@ -170,15 +170,26 @@ __device__ __forceinline__ float cell_update(int flags, int r, int c, int ny, in
 
 // accumulate_qs_x / accumulate_qs_y (sediment.py:218-273) for one face: the MPM
 // capacity at the face's flow depth, Kahan-added (times dt) to the integral at `o`.
-template <typename FloatPtr>
+//
+// Most faces of a run carry no bedload (dry, still, or under the threshold): the d50
+// pair is read only when the face is wet and moving, and a zero increment onto a zero
+// compensation -- kahan_add of +-0 leaves both the integral (never -0: it starts at
+// +0) and the +0 compensation exactly as they are -- reads one value and writes
+// nothing.  The bits are those of the unconditional update.
+template <typename DPtr, typename FloatPtr>
 __device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_b, float z_a,
-                                                float z_b, float n_a, float n_b, float d_a,
-                                                float d_b, float dt, FloatPtr integ, FloatPtr debt,
-                                                int64_t o) {
+                                                float z_b, float n_a, float n_b, DPtr d50,
+                                                int64_t oa, int64_t ob, float dt,
+                                                FloatPtr integ, FloatPtr debt, int64_t o) {
   const float h_face = fmx(eta_a, eta_b) - fmx(z_a, z_b);
-  const float qs = face_capacity(q, h_face, h_face, 0.5f * (n_a + n_b), 0.5f * (d_a + d_b));
-  float si = integ[o], sc = debt[o];
-  kahan_add(si, sc, qs * dt);
+  float qs = 0.0f;  // face_capacity's dry and still exits, taken before the d50 reads
+  if (!(h_face < kHDry) && q != 0.0f)
+    qs = face_capacity(q, h_face, h_face, 0.5f * (n_a + n_b), 0.5f * (d50[oa] + d50[ob]));
+  const float inc = qs * dt;
+  float sc = debt[o];
+  if (inc == 0.0f && sc == 0.0f) return;
+  float si = integ[o];
+  kahan_add(si, sc, inc);
   integ[o] = si;
   debt[o] = sc;
 }
@@ -334,11 +345,11 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
         if (c >= 1)
           accumulate_face(qxL, s_eta[lr + 2][lc + 1], s_eta[lr + 2][lc + 2], s_z[lr + 2][lc + 1],
                           s_z[lr + 2][lc + 2], s_n[lr + 2][lc + 1], s_n[lr + 2][lc + 2],
-                          a.d50[o - 1], a.d50[o], k.dt, a.qs_int_x, a.qs_comp_x, o);
+                          a.d50, o - 1, o, k.dt, a.qs_int_x, a.qs_comp_x, o);
         if (r >= 1)
           accumulate_face(qyT, s_eta[lr + 1][lc + 2], s_eta[lr + 2][lc + 2], s_z[lr + 1][lc + 2],
                           s_z[lr + 2][lc + 2], s_n[lr + 1][lc + 2], s_n[lr + 2][lc + 2],
-                          a.d50[o - P], a.d50[o], k.dt, a.qs_int_y, a.qs_comp_y, o);
+                          a.d50, o - P, o, k.dt, a.qs_int_y, a.qs_comp_y, o);
       }
       float comp = (flags & LI_COMP) ? a.h_comp[o] : 0.0f;
       const float rain_v = (flags & LI_FIELD) ? a.rain[o] : 0.0f;
@@ -600,11 +611,11 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         const int64_t o = row + c;
         if (SED && (flags & LI_SED)) {
           if (c >= 1)
-            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, glob(a.d50)[o - 1], glob(a.d50)[o],
-                            k.dt, glob(a.qs_int_x), glob(a.qs_comp_x), o);
+            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, glob(a.d50), o - 1, o, k.dt,
+                            glob(a.qs_int_x), glob(a.qs_comp_x), o);
           if (i >= 1)
-            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, glob(a.d50)[o - P], glob(a.d50)[o],
-                            k.dt, glob(a.qs_int_y), glob(a.qs_comp_y), o);
+            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, glob(a.d50), o - P, o, k.dt,
+                            glob(a.qs_int_y), glob(a.qs_comp_y), o);
         }
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
