@@ -20,10 +20,15 @@ struct rbs_batch {
   int capacity = 0;
   std::vector<rbs_state *> slot;  // slot -> member (nullptr = free)
   hipStream_t stream = nullptr;
+  // the bedload and wide segments' launches run on side streams, forked from and
+  // joined to `stream` inside the captured graph, beside the plain segment's
+  hipStream_t side = nullptr, side_wide = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, join_wide_ev = nullptr;
   int variant = li_default_variant();          // members without the bedload integral
   int variant_sed = li_default_sed_variant();  // members with it
   int chunk = 16;
-  int plain_tiles = 0;   // tiles of members without the bedload integral (listed first)
+  // wide members without the bedload integral (at least li_wide_batch_min_nx()
+  // columns) run the row-block kernel in their own segment, as they would alone
   // device tables (rebuilt when membership changes)
   void *d_args = nullptr;      // LiArgs[capacity]
   FinArgs *d_fin = nullptr;    // [capacity]
@@ -37,10 +42,10 @@ struct rbs_batch {
   BatchDims *d_dims = nullptr;  // the live counts the captured kernels read
   char *h_stage = nullptr;      // pinned staging of the tables (one rebuild's worth)
   size_t stage_cap = 0;
-  bool any_sed = false, any_inflow = false;
+  bool any_sed = false, any_wide = false, any_inflow = false;
   // what the current graph was captured for: launch sizes and optional nodes
   int graph_tiles = 0;
-  bool graph_sed = false, graph_inflow = false;
+  bool graph_sed = false, graph_wide = false, graph_inflow = false;
   std::vector<int> order;      // compact index -> slot
   std::vector<int> active;     // per slot: 1 running, 0 done
   hipGraphExec_t graph = nullptr;
@@ -71,27 +76,30 @@ static int rebuild(rbs_batch *b) {
     if (b->slot[k] && b->active[k]) b->order.push_back(k);
   const int n = (int)b->order.size();
   b->total_tiles = 0;
-  b->plain_tiles = 0;
   b->n_inflow = 0;
   b->dirty = false;
   const size_t asz = li_args_size();
   std::vector<char> args(asz * (n > 0 ? n : 1));
   std::vector<FinArgs> fin(n > 0 ? n : 1);
-  // two segments of the flat tile space, each launched with its own kernel form:
-  // members without the bedload integral first, then those with it; first[i] is
-  // member i's first tile within its segment
-  std::vector<int> first(n > 0 ? n : 1), member, member_sed;
+  // three segments of the flat tile space, each launched with its own kernel: plain
+  // members on the batch's variant, wide plain members on the row blocks, then the
+  // members with the bedload integral; first[i] is member i's first tile within its
+  // segment
+  std::vector<int> first(n > 0 ? n : 1), seg_member[kBatchSegments];
   std::vector<InflowArgs> inflow;
   for (int i = 0; i < n; ++i) {
     rbs_state *s = b->slot[b->order[i]];
     const bool sed = (s->armed & RBS_ARM_SED) != 0;
-    s->li_variant = sed ? b->variant_sed : b->variant;
+    const bool wide = !sed && s->nx >= li_wide_batch_min_nx();
+    const int sg = sed ? 1 : (wide ? 2 : 0);
+    s->li_variant = sg == 1 ? b->variant_sed : (sg == 2 ? li_wide_batch_variant() : b->variant);
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
-    std::vector<int> &seg = sed ? member_sed : member;
+    std::vector<int> &seg = seg_member[sg];
     first[i] = (int)seg.size();
     for (int t = 0; t < nt; ++t) seg.push_back(i);
     if (sed) b->any_sed = true;
+    if (wide) b->any_wide = true;
     if (s->inflow.count > 0) {
       inflow.emplace_back();
       inflow_fill_args(s, &inflow.back());
@@ -106,9 +114,20 @@ static int rebuild(rbs_batch *b) {
     fin[i].nblk = nt;
   }
   b->n_inflow = (int)inflow.size();
-  b->plain_tiles = (int)member.size();
-  b->total_tiles = (int)(member.size() + member_sed.size());
-  member.insert(member.end(), member_sed.begin(), member_sed.end());
+  // flat order: plain, wide, bedload
+  BatchDims dims{};
+  dims.seg_first[0] = 0;
+  dims.seg_tiles[0] = (int)seg_member[0].size();
+  dims.seg_first[2] = dims.seg_tiles[0];
+  dims.seg_tiles[2] = (int)seg_member[2].size();
+  dims.seg_first[1] = dims.seg_first[2] + dims.seg_tiles[2];
+  dims.seg_tiles[1] = (int)seg_member[1].size();
+  dims.members = n;
+  dims.inflow_members = b->n_inflow;
+  std::vector<int> member = seg_member[0];
+  member.insert(member.end(), seg_member[2].begin(), seg_member[2].end());
+  member.insert(member.end(), seg_member[1].begin(), seg_member[1].end());
+  b->total_tiles = (int)member.size();
   if (b->total_tiles > b->tile_cap) {  // grow by half again: few reallocations
     RBS_HIP(hipStreamSynchronize(b->stream));
     if (b->d_tile_member) RBS_HIP(hipFree(b->d_tile_member));
@@ -119,9 +138,8 @@ static int rebuild(rbs_batch *b) {
     drop_graph(b);  // it holds the old table
   }
   if (b->graph && (b->total_tiles > b->graph_tiles || (b->any_sed && !b->graph_sed) ||
-                   (b->any_inflow && !b->graph_inflow)))
+                   (b->any_wide && !b->graph_wide) || (b->any_inflow && !b->graph_inflow)))
     drop_graph(b);
-  const BatchDims dims{b->plain_tiles, b->total_tiles, n, b->n_inflow};
   // stage every table in pinned memory, then copy them with DMA in stream order
   struct Piece {
     void *dev;
@@ -157,21 +175,43 @@ static int rebuild(rbs_batch *b) {
 }
 
 // Capture `chunk` batched steps with launch sizes covering the tile capacity and the
-// member capacity; the kernels read the live counts from d_dims.
+// member capacity; the kernels read the live counts from d_dims.  The segments touch
+// disjoint members, so the bedload and wide segments' launches fork off to side
+// streams and join before the finalize: in the graph the three run side by side, and
+// a step of a mixed batch takes the longest of them rather than their sum.
 static int capture(rbs_batch *b) {
   const int grid_tiles = b->tile_cap;
-  const bool sed = b->any_sed, inflow = b->any_inflow;
+  const bool sed = b->any_sed, wide = b->any_wide, inflow = b->any_inflow;
   RBS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBS_OK;
+  auto ok = [&](hipError_t e, const char *what) {
+    if (rc == RBS_OK && e != hipSuccess) rc = hip_fail(e, what);
+  };
   for (int k = 0; k < b->chunk && rc == RBS_OK; ++k) {
     // point sources of the members that have them (inflow.py:167-200), one launch
     if (inflow) rc = inflow_batch_launch(b->d_inflow, b->d_dims, b->capacity, b->stream);
+    if (rc == RBS_OK && (sed || wide))
+      ok(hipEventRecord(b->fork_ev, b->stream), "hipEventRecord(fork)");
+    if (rc == RBS_OK && sed) {
+      ok(hipStreamWaitEvent(b->side, b->fork_ev, 0), "hipStreamWaitEvent(fork)");
+      if (rc == RBS_OK)
+        rc = li_batch_launch(b->variant_sed, 1, (const LiArgs *)b->d_args, b->d_tile_member,
+                             b->d_tile_first, b->d_dims, grid_tiles, b->side);
+      ok(hipEventRecord(b->join_ev, b->side), "hipEventRecord(join)");
+    }
+    if (rc == RBS_OK && wide) {
+      ok(hipStreamWaitEvent(b->side_wide, b->fork_ev, 0), "hipStreamWaitEvent(fork)");
+      if (rc == RBS_OK)
+        rc = li_batch_launch(li_wide_batch_variant(), 2, (const LiArgs *)b->d_args,
+                             b->d_tile_member, b->d_tile_first, b->d_dims, grid_tiles,
+                             b->side_wide);
+      ok(hipEventRecord(b->join_wide_ev, b->side_wide), "hipEventRecord(join)");
+    }
     if (rc == RBS_OK)
-      rc = li_batch_launch(b->variant, false, (const LiArgs *)b->d_args, b->d_tile_member,
+      rc = li_batch_launch(b->variant, 0, (const LiArgs *)b->d_args, b->d_tile_member,
                            b->d_tile_first, b->d_dims, grid_tiles, b->stream);
-    if (rc == RBS_OK && sed)
-      rc = li_batch_launch(b->variant_sed, true, (const LiArgs *)b->d_args, b->d_tile_member,
-                           b->d_tile_first, b->d_dims, grid_tiles, b->stream);
+    if (sed) ok(hipStreamWaitEvent(b->stream, b->join_ev, 0), "hipStreamWaitEvent(join)");
+    if (wide) ok(hipStreamWaitEvent(b->stream, b->join_wide_ev, 0), "hipStreamWaitEvent(join)");
     if (rc == RBS_OK)
       rc = batch_finalize_launch(b->d_fin, b->d_dims, b->capacity, b->d_status, b->stream);
   }
@@ -187,6 +227,7 @@ static int capture(rbs_batch *b) {
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   b->graph_tiles = grid_tiles;
   b->graph_sed = sed;
+  b->graph_wide = wide;
   b->graph_inflow = inflow;
   return RBS_OK;
 }
@@ -206,8 +247,14 @@ int rbs_batch_create(int32_t capacity, rbs_batch **out) {
     batch_destroy(b);
     return rc;
   };
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side_wide, hipStreamNonBlocking) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipStreamCreate"));
+  if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join_wide_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(hip_fail(hipErrorUnknown, "hipEventCreate(fork/join)"));
   if (hipEventCreate(&b->start_ev) != hipSuccess || hipEventCreate(&b->done_ev) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipEventCreate"));
   if (hipMalloc(&b->d_args, li_args_size() * capacity) != hipSuccess ||
@@ -240,6 +287,11 @@ static int batch_destroy(rbs_batch *b) {
   if (b->h_stage) (void)hipHostFree(b->h_stage);
   if (b->done_ev) (void)hipEventDestroy(b->done_ev);
   if (b->start_ev) (void)hipEventDestroy(b->start_ev);
+  if (b->join_wide_ev) (void)hipEventDestroy(b->join_wide_ev);
+  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  if (b->side_wide) (void)hipStreamDestroy(b->side_wide);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RBS_OK;

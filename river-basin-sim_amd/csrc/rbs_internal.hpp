@@ -99,9 +99,13 @@ struct Partials {
 // The size of a batch's current membership, read by the batched kernels from device
 // memory: a captured graph launches grids sized for the batch's capacity, the blocks
 // past these counts return at once, and a membership change only rewrites tables.
+// The flat tile space of a batch has three segments, each launched with its own kernel:
+// 0 members without the bedload integral on the batch's plain variant, 1 members with
+// it, 2 wide members without it on the row-block variant.
+constexpr int kBatchSegments = 3;
 struct BatchDims {
-  int32_t plain_tiles;     // tile segment 0: members without the bedload integral
-  int32_t total_tiles;     // segment 1 (bedload members) is [plain_tiles, total_tiles)
+  int32_t seg_first[kBatchSegments];  // the segment's first flat tile
+  int32_t seg_tiles[kBatchSegments];  // its live tile count
   int32_t members;         // finalize blocks
   int32_t inflow_members;  // members with point sources
 };
@@ -225,6 +229,8 @@ int hllc_default_variant();
 void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
+int li_wide_batch_variant();  // the row blocks a batch gives its wide plain members
+int li_wide_batch_min_nx();
 int li_initial_variant(int nx);
 int chan_variant_count();
 int chan_default_variant();
@@ -241,7 +247,7 @@ int batch_finalize_launch(const FinArgs *fin, const BatchDims *dims, int grid, i
 void inflow_fill_args(const rbs_state *s, InflowArgs *out);
 int inflow_batch_launch(const InflowArgs *list, const BatchDims *dims, int grid,
                         hipStream_t stream);
-int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
+int li_batch_launch(int variant, int seg, const LiArgs *members, const int *tile_member,
                     const int *tile_first, const BatchDims *dims, int grid, hipStream_t stream);
 size_t li_args_size();
 void li_fill_args(const rbs_state *s, int limit, LiArgs *out);

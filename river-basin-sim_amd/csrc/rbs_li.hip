@@ -888,8 +888,8 @@ __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_kernel(LiArgs a)
 __device__ __forceinline__ bool batch_tile(const int *__restrict__ tile_member,
                                            const int *__restrict__ tile_first,
                                            const BatchDims *dims, int seg, int &m, int &tile) {
-  const int base = seg ? dims->plain_tiles : 0;
-  const int total = seg ? dims->total_tiles - dims->plain_tiles : dims->plain_tiles;
+  const int base = dims->seg_first[seg];
+  const int total = dims->seg_tiles[seg];
   if ((int)blockIdx.x >= total) return false;
   const int t = xcd_tile(blockIdx.x, total);
   // the member index is block-uniform: its arguments are read with scalar loads
@@ -1243,6 +1243,8 @@ int li_default_variant() { return kDefaultLiVariant; }
 // (where a ragged last block idles few of their waves; 176-180 us against 215 us for
 // the default at 4096^2), the library default otherwise.
 int li_initial_variant(int nx) { return nx >= 2048 ? kRowsVariantForWide : kDefaultLiVariant; }
+int li_wide_batch_variant() { return kRowsVariantForWide; }
+int li_wide_batch_min_nx() { return 2048; }
 // batches of bedload members: strips of 16 rows (measured fastest inside a batch,
 // where the members' tiles fill the GPU whatever their shapes)
 int li_default_sed_variant() { return 6; }
@@ -1255,11 +1257,11 @@ void li_variant_shape(int v, int *bx, int *by, int *family) {
 
 size_t li_args_size() { return sizeof(LiArgs); }
 
-int li_batch_launch(int variant, bool sed, const LiArgs *members, const int *tile_member,
+int li_batch_launch(int variant, int seg, const LiArgs *members, const int *tile_member,
                     const int *tile_first, const BatchDims *dims, int grid, hipStream_t stream) {
   const LiVariant &v = variant_at(variant);
-  hipLaunchKernelGGL(sed ? v.batch_sed : v.batch, dim3(grid), dim3(v.threads), 0, stream,
-                     members, tile_member, tile_first, dims, sed ? 1 : 0);
+  hipLaunchKernelGGL(seg == 1 ? v.batch_sed : v.batch, dim3(grid), dim3(v.threads), 0, stream,
+                     members, tile_member, tile_first, dims, seg);
   RBS_HIP(hipGetLastError());
   return RBS_OK;
 }

@@ -95,3 +95,29 @@ def test_burst_builds_overlap_stepping_and_results_equal_solo():
         alone = solo(req)
         assert by_id[req.rid].steps == alone.steps
         assert np.array_equal(by_id[req.rid].h, alone.h), f"request {req.rid} ({req.kind})"
+
+
+def test_narrow_wide_and_bedload_segments_equal_solo_results():
+    """A batch holding all three tile segments at once: narrow basins on the strips, a
+    basin at least 2048 columns wide on the row blocks, and reaches with the bedload
+    integral, each segment on its own graph branch; every result equals its solo run."""
+    from rbs.serve import InputPool, Server, basin_request, reach_request, solo
+
+    pool = InputPool(n=2112, seed=5)
+    rng = np.random.default_rng(5)
+    # the small requests run long enough to still be live while the wide one is built
+    reqs = [basin_request(0, 0.0, pool, rng, n_min=96, n_max=200, end_time=3600.0),
+            basin_request(1, 0.0, pool, rng, n_min=2050, n_max=2100, end_time=300.0),
+            reach_request(2, 0.0, pool, rng, end_time=3600.0, scale=0.125),
+            basin_request(3, 0.01, pool, rng, n_min=96, n_max=200, end_time=3600.0)]
+    reqs[2].interval_s = 600.0
+    assert reqs[1].bed.shape[1] >= 2048 > max(r.bed.shape[1] for r in reqs if r.rid != 1)
+    srv = Server(capacity=8, chunk=16, keep_results=True)
+    stats = srv.serve(reqs)
+    assert stats["completed"] == len(reqs) and stats["peak_live"] >= 3
+    by_id = {r.rid: r for r in srv.results}
+    for req in reqs:
+        alone = solo(req)
+        got = by_id[req.rid]
+        assert got.steps == alone.steps
+        assert np.array_equal(got.h, alone.h), f"request {req.rid} ({req.kind}) differs"
