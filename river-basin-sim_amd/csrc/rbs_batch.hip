@@ -92,7 +92,13 @@ static int rebuild(rbs_batch *b) {
     const bool sed = (s->armed & RBS_ARM_SED) != 0;
     const bool wide = !sed && s->nx >= li_wide_batch_min_nx();
     const int sg = sed ? 1 : (wide ? 2 : 0);
-    s->li_variant = sg == 1 ? b->variant_sed : (sg == 2 ? li_wide_batch_variant() : b->variant);
+    const int v = sg == 1 ? b->variant_sed : (sg == 2 ? li_wide_batch_variant() : b->variant);
+    if (v != s->li_variant) {
+      // another tiling: clear the dry-tile words in the batch's stream order
+      const int rc = quiet_reset(s, b->stream);
+      if (rc) return rc;
+    }
+    s->li_variant = v;
     li_fill_args(s, 1, reinterpret_cast<LiArgs *>(args.data() + asz * i));
     const int nt = li_num_blocks(s);
     std::vector<int> &seg = seg_member[sg];

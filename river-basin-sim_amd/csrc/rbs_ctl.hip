@@ -233,12 +233,20 @@ int ensure_partials(rbs_state *s, int nblk) {
     RBS_HIP(hipFree(s->part.vol));
     RBS_HIP(hipFree(s->part.infil));
     RBS_HIP(hipFree(s->part.edge));
+    RBS_HIP(hipFree(s->quiet));
   }
   RBS_HIP(hipMalloc(&s->part.speed_max, sizeof(float) * nblk));
   RBS_HIP(hipMalloc(&s->part.vol, sizeof(double) * nblk));
   RBS_HIP(hipMalloc(&s->part.infil, sizeof(double) * nblk));
   RBS_HIP(hipMalloc(&s->part.edge, sizeof(double) * nblk));
+  RBS_HIP(hipMalloc(&s->quiet, sizeof(uint32_t) * nblk));
   s->part_cap = nblk;
+  return quiet_reset(s, s->stream);
+}
+
+int quiet_reset(rbs_state *s, hipStream_t stream) {
+  if (s->quiet && s->part_cap > 0)
+    RBS_HIP(hipMemsetAsync(s->quiet, 0, sizeof(uint32_t) * s->part_cap, stream));
   return RBS_OK;
 }
 

@@ -188,6 +188,9 @@ struct rbs_state {
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;
+  // per LI tile: which buffers and compensations hold only +0 (the strip kernels
+  // skip dry tiles with it); part_cap words, cleared by quiet_reset
+  uint32_t *quiet = nullptr;
   rbs::DevCtl *ctl = nullptr;        // device
   rbs::DevCtl *ctl_host = nullptr;   // pinned host mirror for status reads
   double *d_sync = nullptr;
@@ -239,6 +242,10 @@ int chan_num_blocks(const rbs_state *s);
 int li_default_sed_variant();
 void li_variant_shape(int v, int *bx, int *by, int *family);
 int ensure_partials(rbs_state *s, int nblk);
+// Clear the dry-tile words (stream-ordered on `stream`): every tile computes its next
+// two steps.  Needed whenever h, qx, qy or a Kahan debt is written outside the LI step,
+// or the tiling changes.
+int quiet_reset(rbs_state *s, hipStream_t stream);
 
 int driven_setup(rbs_state *s, const rbs_run_params *p, double t_stop, int64_t max_steps,
                  int64_t *steps0);

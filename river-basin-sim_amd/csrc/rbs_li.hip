@@ -55,6 +55,7 @@ struct LiArgs {
   float *qx_ch[2], *qx_fp[2], *qy_ch[2], *qy_fp[2];
   const DevCtl *ctl;
   Partials part;
+  uint32_t *quiet;  // per tile: which of its buffers and compensations are all +0
   double dx64;
   int64_t pitch;
   int32_t ny, nx;
@@ -175,9 +176,10 @@ __device__ __forceinline__ float cell_update(int flags, int r, int c, int ny, in
 // pair is read only when the face is wet and moving, and a zero increment onto a zero
 // compensation -- kahan_add of +-0 leaves both the integral (never -0: it starts at
 // +0) and the +0 compensation exactly as they are -- reads one value and writes
-// nothing.  The bits are those of the unconditional update.
+// nothing.  The bits are those of the unconditional update.  Returns the bits of the
+// face's compensation after the update.
 template <typename DPtr, typename FloatPtr>
-__device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_b, float z_a,
+__device__ __forceinline__ uint32_t accumulate_face(float q, float eta_a, float eta_b, float z_a,
                                                 float z_b, float n_a, float n_b, DPtr d50,
                                                 int64_t oa, int64_t ob, float dt,
                                                 FloatPtr integ, FloatPtr debt, int64_t o) {
@@ -187,11 +189,12 @@ __device__ __forceinline__ void accumulate_face(float q, float eta_a, float eta_
     qs = face_capacity(q, h_face, h_face, 0.5f * (n_a + n_b), 0.5f * (d50[oa] + d50[ob]));
   const float inc = qs * dt;
   float sc = debt[o];
-  if (inc == 0.0f && sc == 0.0f) return;
+  if (inc == 0.0f && sc == 0.0f) return __float_as_uint(sc);
   float si = integ[o];
   kahan_add(si, sc, inc);
   integ[o] = si;
   debt[o] = sc;
+  return __float_as_uint(sc);
 }
 
 template <int NT = kThreads>
@@ -441,6 +444,19 @@ __device__ __forceinline__ float from_right(float v) {
 // costs the HBM far more than the bytes (the LI step's store pattern of 60-column
 // strips takes about 11% longer than the same bytes stored in whole lines).  One barrier per row keeps the
 // double-buffered LDS rows in step.
+//
+// Dry tiles.  A tile whose depths, one cell out included, are all +0 at a step with no
+// rain has every face dry (face depth max(z_a, z_b) - max(z_a, z_b) = 0 < kHDry), so
+// its faces, depths, compensations and partials come out +0 whatever the old faces
+// were.  Such a tile can skip the step if the output buffer already holds +0 in its
+// cells and its in-place compensations are +0: then nothing it would write differs
+// from what memory holds.  The per-tile flag `quiet` keeps that knowledge: set by a
+// computed step that read and wrote only +0 (h, qx, qy of both buffers and the Kahan
+// debts of the tile's cells), cleared by one that did not, left by a skipped step.
+// Writers outside the step touch only the current buffer's depths (point inflows,
+// reservoirs), which the skip reads itself, one cell out, before relying on the flag;
+// anything else (uploads of h, q or debts, a change of tiling) clears the flags.
+
 template <int H, bool SED, int NW = 0, int FL = 1>
 __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   // FL: finished rows stored per barrier, from a ring of 2 FL LDS rows (two rows per
@@ -498,6 +514,30 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   const bool col_ok = c >= 0 && c < nx;
   const bool xface_ok = c >= 1 && c <= nx - 1;  // interior x-face left of column c
   const bool own = lane >= 2 && lane < 2 + kStripW && col_ok;
+
+  // dry tiles are skipped by the 60-column strips (the row blocks, the kernel of wide
+  // rain-on-terrain grids, keep their registers and compute every tile)
+  constexpr bool DRY = !ROWS;
+  const gptr<uint32_t> quiet = glob(a.quiet);
+  const bool no_rain = k.rain == 0.0f && (!(flags & LI_FIELD) || k.rain_scale == 0.0f);
+  if (DRY && no_rain && quiet[tile] != 0u) {  // block-uniform
+    uint32_t wet = 0;
+    if (y0 < ny)  // the tile's rows and one row either side, 64 columns per wave
+      for (int r = y0 - 1; r <= y_end; ++r)
+        wet |= __float_as_uint(at(h_in + (int64_t)clampi(r, 0, ny - 1) * P, cc));
+    if (!__syncthreads_or(wet != 0u)) {
+      if (threadIdx.x == 0) {
+        glob(a.part.speed_max)[tile] = 0.0f;
+        glob(a.part.vol)[tile] = 0.0;
+        glob(a.part.infil)[tile] = 0.0;
+        glob(a.part.edge)[tile] = 0.0;
+      }
+      return;
+    }
+  }
+  // the wave's lanes that read or wrote anything but +0 (h, qx, qy in and out, the
+  // debts): ballots, so the mask stays in scalar registers
+  uint64_t nz = 0;
 
   // row blocks: the finished row i (iteration it) sits in LDS row slot it % (2 FL);
   // after every FL-th row of the block (and its last), one barrier, then wave w stores
@@ -575,6 +615,9 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     for (int it = 1; it < H + 4; ++it) {
       const int R = y0 - 2 + it;
       const float h0 = ph, z0 = pz, n0 = pn, qxo = pqx, qyo = pqy;
+      if (DRY && R >= y0 && R < y_end)  // wave-uniform
+        nz |= __ballot(own && (__float_as_uint(h0) | __float_as_uint(qxo) |
+                               __float_as_uint(qyo)) != 0u);
       float comp = pcomp;
       const float rain_v = prain, infil_v = pinfil;
       if (it + 1 < H + 4) ingest(R + 1);
@@ -604,6 +647,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         n2l = from_left(n2);
       }
       const bool row_live = i >= y0 && i < y_end;  // block-uniform for ROWS
+      uint32_t bits = 0;  // what the lane writes (and the debts it leaves)
       if (row_live && own) {
         float qxL = fx2, qxR = fx2r, qyT = fy2, qyB = fy1;
         if (limit) limit_faces(qxL, qxR, qyT, qyB, b2, b2l, b2r, b3, b1);
@@ -611,14 +655,16 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         const int64_t o = row + c;
         if (SED && (flags & LI_SED)) {
           if (c >= 1)
-            accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, glob(a.d50), o - 1, o, k.dt,
-                            glob(a.qs_int_x), glob(a.qs_comp_x), o);
+            bits |= accumulate_face(qxL, e2l, e2, z2l, z2, n2l, n2, glob(a.d50), o - 1, o,
+                                    k.dt, glob(a.qs_int_x), glob(a.qs_comp_x), o);
           if (i >= 1)
-            accumulate_face(qyT, e3, e2, z3, z2, n3, n2, glob(a.d50), o - P, o, k.dt,
-                            glob(a.qs_int_y), glob(a.qs_comp_y), o);
+            bits |= accumulate_face(qyT, e3, e2, z3, z2, n3, n2, glob(a.d50), o - P, o, k.dt,
+                                    glob(a.qs_int_y), glob(a.qs_comp_y), o);
         }
         const float h = cell_update(flags, i, c, ny, nx, h2, qxL, qxR, qyT, qyB, k, comp,
                                     rain_v, infil_v, sum);
+        bits |= __float_as_uint(h) | __float_as_uint(qxL) | __float_as_uint(qyT);
+        if (flags & LI_COMP) bits |= __float_as_uint(comp);
         if (ROWS) {
           const int j = kStripW * wave + lane - 2;  // in [0, BWC)
           const int slot = it % (2 * FL);
@@ -635,6 +681,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
         if (c == nx - 1) at(qx_out + row, cu + 1) = qxR;
         if (i == ny - 1) at(qy_out + row + P, cu) = qyB;
       }
+      if (DRY && row_live) nz |= __ballot(bits != 0u);  // wave-uniform
       if (ROWS && row_live) flush_rows(it, i);
       // slide the window
       if (SED) {
@@ -649,6 +696,11 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     }
   }
   write_partials<NT>(a, tile, sum, s_red);
+  // the tile's flag for the next steps (with rain falling no tile is skipped: 0)
+  if (DRY) {
+    const bool still = no_rain && !__syncthreads_or(nz != 0);
+    if (threadIdx.x == 0) quiet[tile] = still ? 1u : 0u;
+  }
 }
 
 // ---- family 3: LDS tile of a state with sub-grid channels --------------------------
@@ -1323,6 +1375,7 @@ void li_fill_args(const rbs_state *s, int limit, LiArgs *out) {
   }
   a.ctl = s->ctl;
   a.part = s->part;
+  a.quiet = s->quiet;
   a.dx64 = s->dx;
   a.pitch = s->pitch;
   a.ny = s->ny;

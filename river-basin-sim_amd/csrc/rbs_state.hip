@@ -330,7 +330,7 @@ static int state_destroy(rbs_state *s) {
                   s->h_comp, s->rain, s->infil, s->hu, s->hv, s->h1, s->hu1, s->hv1,
                   s->d50, s->qs_int_x, s->qs_comp_x, s->qs_int_y, s->qs_comp_y, s->z0,
                   s->dz_lo, s->dz_hi, s->dz_cum, s->dz_unapplied, s->sed_part, s->part.speed_max,
-                  s->part.vol, s->part.infil, s->part.edge, s->ctl, s->d_sync,
+                  s->part.vol, s->part.infil, s->part.edge, s->quiet, s->ctl, s->d_sync,
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
                   s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol, s->ch_w, s->ch_d,
                   s->ch_n, s->qx_ch[0], s->qx_ch[1], s->qx_fp[0], s->qx_fp[1], s->qy_ch[0],
@@ -443,6 +443,11 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
   }
   rc = copy_rows(s, v, const_cast<void *>(host), true);
   if (rc) return rc;
+  if (field == RBS_F_H || field == RBS_F_QX || field == RBS_F_QY || field == RBS_F_H_COMP ||
+      field == RBS_F_QS_COMP_X || field == RBS_F_QS_COMP_Y) {
+    rc = quiet_reset(s, s->stream);
+    if (rc) return rc;
+  }
   if (field == RBS_F_H || field == RBS_F_HU || field == RBS_F_HV || field == RBS_F_CH_W ||
       field == RBS_F_CH_D)
     s->speed_scheme = -1;
@@ -678,6 +683,10 @@ int rbs_set_li_variant(rbs_state *s, int32_t variant) {
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (variant < 0 || variant >= li_variant_count())
     return set_error("unknown LI variant %d", variant), RBS_E_ARG;
+  if (variant != s->li_variant) {
+    const int rc = quiet_reset(s, s->stream);  // another tiling
+    if (rc) return rc;
+  }
   s->li_variant = variant;
   invalidate_graph(s);
   return RBS_OK;
