@@ -155,7 +155,8 @@ int rbs_state_bytes(const rbs_state *st, uint64_t *bytes);
 
 /* wp.array(...) / .assign(...) and .numpy() on State fields (state.py:200-226).
  * `bytes` must equal the field's dense size.  Uploading h, z or hu/hv marks the
- * cached CFL reduction stale. */
+ * cached CFL reduction stale; uploading h, qx, qy or a Kahan debt clears the strip
+ * kernels' dry-tile flags (every tile computes its next steps). */
 int rbs_upload(rbs_state *st, int32_t field, const void *host, size_t bytes);
 int rbs_download(rbs_state *st, int32_t field, void *host, size_t bytes);
 
@@ -330,7 +331,9 @@ int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is
 int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
 /* The LI kernel variants a batch launches: `variant` for members without the bedload
  * integral, `variant_sed` for members with it (library defaults unless set); takes
- * effect at the next advance. */
+ * effect at the next advance.  Members without the bedload integral that are at
+ * least 2048 cells wide run the 16-row row blocks whatever `variant` says, as they
+ * would alone; the three groups launch side by side in the batch's graph. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed);
 int rbs_batch_get_variant(rbs_batch *b, int32_t *variant, int32_t *variant_sed);
 /* Host-side cost of membership changes so far (table rebuilds and graph captures:
