@@ -111,7 +111,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="li",
                     choices=["li", "ensemble", "serve", "hllc", "copy", "channels", "output",
-                             "reservoir"])
+                             "reservoir", "reach"])
     ap.add_argument("--sweep", action="store_true",
                     help="also time every LI kernel variant on the spun-up state")
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
@@ -147,6 +147,8 @@ def main() -> int:
         return bench_channels(args)
     if args.workload == "output":
         return bench_output(args)
+    if args.workload == "reach":
+        return bench_reach(args)
     if args.workload == "reservoir":
         return bench_reservoir(args)
     st, d = build_state(args.n, args.seed)
@@ -700,6 +702,86 @@ def bench_reservoir(args) -> int:
         "wall_s": {"without": round(base, 3), "with": round(wall, 3)},
         "overhead_ms_per_activation": round((wall - base) * 1e3 / max(len(rel), 1), 2),
         "released_m3": [round(r["volume_m3"], 1) for r in recs],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_reach(args) -> int:
+    """The largest served alluvial reach run alone: 4096 x 1024 cells of 2 m, LI with
+    the bedload integral, a point-inflow hydrograph on the channel and morphology
+    stops every 900 s, 1800 s simulated, device-driven: with the 16-row row blocks
+    (which compute every tile), and with the 64 x 8 LDS tile and the 16-row strip
+    (which skip dry floodplain tiles); then every variant's step kernel timed on the
+    final state.  `value` is the faster skipping run."""
+    from rbs import _lib
+    from rbs.serve import InputPool, build_state, reach_request
+
+    pool = InputPool(n=1024, seed=args.seed)
+    req = reach_request(0, 0.0, pool, np.random.default_rng(args.seed), end_time=1800.0,
+                        factor=4.0)
+    ny, nx = req.bed.shape
+    cells = ny * nx
+    count = ctypes.c_int32()
+    _lib.call("rbs_li_variant_count", ctypes.byref(count))
+    info = []
+    for v in range(count.value):
+        bx, by, fam, dflt = (ctypes.c_int32() for _ in range(4))
+        _lib.call("rbs_li_variant_info", v, ctypes.byref(bx), ctypes.byref(by),
+                  ctypes.byref(fam), ctypes.byref(dflt))
+        info.append((v, ("lds_tile", "strip", "strip2", "rows")[fam.value], bx.value, by.value))
+    tile_v = next(v for v, f, bx, by in info if f == "lds_tile" and (bx, by) == (64, 8))
+    strip_v = next(v for v, f, bx, by in info if f == "strip" and by == 16 * 4)
+    rows_v = next(v for v, f, bx, by in info if f == "rows" and by == 16)
+
+    runs, sweep = {}, []
+    for label, variant in (("rows_16", rows_v), ("lds_tile_64x8", tile_v),
+                           ("strip_16", strip_v)):
+        st, run, _ = build_state(req, tune_kernel=False)
+        _lib.call("rbs_set_li_variant", st._handle, ctypes.c_int32(variant))
+        st.sync()
+        t0 = time.perf_counter()
+        steps = 0
+        for seg in run.sched.segments():
+            steps += run.run_until(seg.t_stop)
+            for proc, el in seg.due:
+                proc.advance(seg.t_stop, el)
+        st.sync()
+        wall = time.perf_counter() - t0
+        h = st.h.numpy()
+        runs[label] = {"variant": variant, "steps": steps, "wall_s": round(wall, 3),
+                       "gcell_updates_per_s": round(cells * steps / wall / 1e9, 2),
+                       "dry_cell_fraction": round(float((h == 0).mean()), 4)}
+        log(f"[reach] {label}: {steps} steps in {wall:.3f} s, "
+            f"{cells * steps / wall / 1e9:.1f} Gcell-updates/s, dry {float((h == 0).mean()):.3f}")
+        if label == "strip_16":  # every variant's kernel on the final state, no rain
+            dt = st.ledger().dt
+            for v, fam, bx, by in info:
+                _lib.call("rbs_set_li_variant", st._handle, ctypes.c_int32(v))
+                avg = ctypes.c_double()
+                _lib.call("rbs_time_li_kernel", st._handle, args.kernel_reps, dt, 0.0, 0.0,
+                          ctypes.byref(avg))
+                sweep.append({"variant": v, "family": fam, "tile": f"{bx}x{by}",
+                              "kernel_us": round(avg.value * 1e3, 2)})
+                log(f"[reach] variant {v} {fam} {bx}x{by}: {avg.value * 1e3:.1f} us")
+        st.close()
+    same_steps = len({r["steps"] for r in runs.values()}) == 1
+    best = max(("lds_tile_64x8", "strip_16"), key=lambda k: runs[k]["gcell_updates_per_s"])
+    out = {
+        "metric": METRIC,
+        "value": runs[best]["gcell_updates_per_s"],
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": runs[best]["steps"], "warmup": 0,
+        "ms_per_step": round(runs[best]["wall_s"] * 1e3 / runs[best]["steps"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"li_exner_reach_{ny}x{nx}", "grid": f"{ny}x{nx}", "dx_m": 2.0,
+                   "sim_s": 1800.0, "sources": "point inflow hydrograph, bedload integral, "
+                   "morphology every 900 s, open south edge"},
+        "kernel": best,
+        "runs": runs,
+        "same_steps": same_steps,
+        "kernel_sweep_final_state": sweep,
     }
     print(json.dumps(out), flush=True)
     return 0

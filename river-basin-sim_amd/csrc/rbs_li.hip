@@ -249,6 +249,29 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
   const float dx = (float)a.dx64;
   const float rdx = 1.0f / dx;  // IEEE division: RN(1/dx)
 
+  // dry tiles (the strip kernels' rule): the flag, then the depths one cell out
+  const bool no_rain = k.rain == 0.0f && (!(flags & LI_FIELD) || k.rain_scale == 0.0f);
+  if (no_rain && a.quiet[tile] != 0u) {  // block-uniform
+    uint32_t wet = 0;
+    for (int kk = tid; kk < RH * RW; kk += kThreads) {
+      const int lr = kk / RW, lc = kk - (kk / RW) * RW;
+      const int gi = clampi(y0 + lr - 1, 0, ny - 1), gj = clampi(x0 + lc - 1, 0, nx - 1);
+      wet |= __float_as_uint(h_in[(int64_t)gi * P + gj]);
+    }
+    if (!__syncthreads_or(wet != 0u)) {
+      if (tid == 0) {
+        a.part.speed_max[tile] = 0.0f;
+        a.part.vol[tile] = 0.0;
+        a.part.infil[tile] = 0.0;
+        a.part.edge[tile] = 0.0;
+      }
+      return;
+    }
+  }
+  // bits of everything the thread read (own cells' h, qx, qy) or wrote (the same and
+  // the debts) that was not +0
+  uint32_t nz = 0;
+
   // ---- phase A: stage the halo'd tile in LDS (clamped reads stay in bounds) ----
 #pragma unroll
   for (int k0 = 0; k0 < EH * EW; k0 += kThreads) {
@@ -258,6 +281,8 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
       const int gi = clampi(y0 + lr - 2, 0, ny - 1), gj = clampi(x0 + lc - 2, 0, nx - 1);
       const int64_t o = (int64_t)gi * P + gj;
       const float hv = h_in[o], zv = a.z[o];
+      if (lr >= 2 && lr < BY + 2 && lc >= 2 && lc < BX + 2 && y0 + lr - 2 < ny && x0 + lc - 2 < nx)
+        nz |= __float_as_uint(hv);  // an own cell
       s_eta[lr][lc] = hv + zv;  // compute_eta (:61-69)
       s_z[lr][lc] = zv;
       s_n[lr][lc] = a.n[o];
@@ -270,7 +295,10 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
     if (kk < RH * QXW) {
       const int lr = kk / QXW, lf = kk - (kk / QXW) * QXW;
       const int gi = clampi(y0 + lr - 1, 0, ny - 1), gf = clampi(x0 + lf - 1, 0, nx);
-      s_qx[lr][lf] = qx_in[(int64_t)gi * P + gf];
+      const float q = qx_in[(int64_t)gi * P + gf];
+      if (lr >= 1 && lr <= BY && lf >= 1 && lf <= BX && y0 + lr - 1 < ny && x0 + lf - 1 < nx)
+        nz |= __float_as_uint(q);  // an own cell's left face
+      s_qx[lr][lf] = q;
     }
   }
 #pragma unroll
@@ -279,7 +307,10 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
     if (kk < QYH * RW) {
       const int lf = kk / RW, lc = kk - (kk / RW) * RW;
       const int gf = clampi(y0 + lf - 1, 0, ny), gj = clampi(x0 + lc - 1, 0, nx - 1);
-      s_qy[lf][lc] = qy_in[(int64_t)gf * P + gj];
+      const float q = qy_in[(int64_t)gf * P + gj];
+      if (lf >= 1 && lf <= BY && lc >= 1 && lc <= BX && y0 + lf - 1 < ny && x0 + lc - 1 < nx)
+        nz |= __float_as_uint(q);  // an own cell's top face
+      s_qy[lf][lc] = q;
     }
   }
   __syncthreads();
@@ -346,13 +377,15 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
       const int64_t o = (int64_t)r * P + c;
       if (flags & LI_SED) {  // own faces only: left (c >= 1) and top (r >= 1)
         if (c >= 1)
-          accumulate_face(qxL, s_eta[lr + 2][lc + 1], s_eta[lr + 2][lc + 2], s_z[lr + 2][lc + 1],
-                          s_z[lr + 2][lc + 2], s_n[lr + 2][lc + 1], s_n[lr + 2][lc + 2],
-                          a.d50, o - 1, o, k.dt, a.qs_int_x, a.qs_comp_x, o);
+          nz |= accumulate_face(qxL, s_eta[lr + 2][lc + 1], s_eta[lr + 2][lc + 2],
+                                s_z[lr + 2][lc + 1], s_z[lr + 2][lc + 2], s_n[lr + 2][lc + 1],
+                                s_n[lr + 2][lc + 2], a.d50, o - 1, o, k.dt, a.qs_int_x,
+                                a.qs_comp_x, o);
         if (r >= 1)
-          accumulate_face(qyT, s_eta[lr + 1][lc + 2], s_eta[lr + 2][lc + 2], s_z[lr + 1][lc + 2],
-                          s_z[lr + 2][lc + 2], s_n[lr + 1][lc + 2], s_n[lr + 2][lc + 2],
-                          a.d50, o - P, o, k.dt, a.qs_int_y, a.qs_comp_y, o);
+          nz |= accumulate_face(qyT, s_eta[lr + 1][lc + 2], s_eta[lr + 2][lc + 2],
+                                s_z[lr + 1][lc + 2], s_z[lr + 2][lc + 2], s_n[lr + 1][lc + 2],
+                                s_n[lr + 2][lc + 2], a.d50, o - P, o, k.dt, a.qs_int_y,
+                                a.qs_comp_y, o);
       }
       float comp = (flags & LI_COMP) ? a.h_comp[o] : 0.0f;
       const float rain_v = (flags & LI_FIELD) ? a.rain[o] : 0.0f;
@@ -360,6 +393,8 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
       const float h = cell_update(flags, r, c, ny, nx, s_h[lr + 1][lc + 1], qxL, qxR, qyT, qyB,
                                   k, comp, rain_v, infil_v, sum);
       if (flags & LI_COMP) a.h_comp[o] = comp;
+      nz |= __float_as_uint(h) | __float_as_uint(qxL) | __float_as_uint(qyT) |
+            __float_as_uint(comp);
       h_out[o] = h;
       qx_out[o] = qxL;
       if (c == nx - 1) qx_out[o + 1] = qxR;
@@ -368,6 +403,8 @@ __device__ __forceinline__ void li_tile(const LiArgs &a, const int tile) {
     }
   }
   write_partials(a, tile, sum, s_red);
+  const bool still = no_rain && !__syncthreads_or(nz != 0u);
+  if (tid == 0) a.quiet[tile] = still ? 1u : 0u;
 }
 
 // ---- family 2: register strips ----------------------------------------------------------
@@ -445,7 +482,8 @@ __device__ __forceinline__ float from_right(float v) {
 // strips takes about 11% longer than the same bytes stored in whole lines).  One barrier per row keeps the
 // double-buffered LDS rows in step.
 //
-// Dry tiles.  A tile whose depths, one cell out included, are all +0 at a step with no
+// Dry tiles (the 60-column strips here and the LDS tiles above follow the same rule).
+// A tile whose depths, one cell out included, are all +0 at a step with no
 // rain has every face dry (face depth max(z_a, z_b) - max(z_a, z_b) = 0 < kHDry), so
 // its faces, depths, compensations and partials come out +0 whatever the old faces
 // were.  Such a tile can skip the step if the output buffer already holds +0 in its

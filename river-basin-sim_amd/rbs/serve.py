@@ -399,15 +399,16 @@ def basin_request(rid: int, arrival: float, pool: InputPool, rng, *, n_min=512, 
 
 
 def reach_request(rid: int, arrival: float, pool: InputPool, rng, *, end_time=3600.0,
-                  scale: float = 1.0) -> Request:
+                  scale: float = 1.0, factor: float | None = None) -> Request:
     """Alluvial reach, dx 2 m, length x width log-uniform between 1024x256 and
     4096x1024: a trapezoidal channel (8 m wide, 1.5 m deep) on slope 1e-3 with 2 m of
     fBm floodplain relief, an inflow hydrograph on the channel's upstream cells,
     Exner + MPM with d50 U[0.5, 4] mm, porosity 0.4, morphological factor 1, open
-    downstream edge.  ``scale`` shrinks both sides (small test cases)."""
+    downstream edge.  ``scale`` shrinks both sides (small test cases); ``factor``
+    fixes the size factor in [1, 4] instead of drawing it."""
     from rbs.processes.inflow import Inflow
 
-    f = np.exp(rng.uniform(0.0, np.log(4.0))) * scale
+    f = (np.exp(rng.uniform(0.0, np.log(4.0))) if factor is None else float(factor)) * scale
     ny, nx = max(int(round(1024 * f)), 16), max(int(round(256 * f)), 16)
     dx = 2.0
     yy = np.arange(ny)[:, None] * np.ones((1, nx))

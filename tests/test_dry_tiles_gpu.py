@@ -1,11 +1,11 @@
-"""Dry tiles: the 60-column strip kernels skip a tile whose depths, one cell out, are
-all +0 at a step without rain, when a per-tile flag says that both ping-pong buffers
-and the Kahan debts of its cells hold +0 -- everything the step would write is then
-already in memory.  The skipping kernels must write the bits of the LDS-tile kernels
-(which compute every cell) on runs that are mostly dry: an alluvial reach with a dry
-floodplain, point inflows, the bedload integral and morphology stops; a basin whose
-rain stops and whose infiltration takes cells back to 0; and runs with an upload and a
-change of tiling in mid-run, which clear the flags.
+"""Dry tiles: the LDS-tile and 60-column strip kernels skip a tile whose depths, one
+cell out, are all +0 at a step without rain, when a per-tile flag says that both
+ping-pong buffers and the Kahan debts of its cells hold +0 -- everything the step
+would write is then already in memory.  The skipping kernels must write the bits of
+the row-block kernels (which compute every tile) on runs that are mostly dry: an
+alluvial reach with a dry floodplain, point inflows, the bedload integral and
+morphology stops; a basin whose rain stops and whose infiltration takes cells back to
+0; and runs with an upload and a change of tiling in mid-run, which clear the flags.
 """
 
 from __future__ import annotations
@@ -21,13 +21,14 @@ LEDGER = ("steps", "t", "volume", "rain_in", "point_in", "infil_out", "edge_out"
 
 
 def _families():
+    """The reference kernel (a row block: computes every tile) and the skipping ones."""
     from rbs.autotune import variant_table
 
     table = variant_table()
-    tile = [r["variant"] for r in table if r["family"] == "lds_tile"]
-    strip = [r["variant"] for r in table if r["family"] == "strip"]
-    assert tile and strip
-    return tile[0], strip
+    rows = [r["variant"] for r in table if r["family"] == "rows"]
+    skipping = [r["variant"] for r in table if r["family"] in ("lds_tile", "strip")]
+    assert rows and skipping
+    return rows[0], skipping
 
 
 def _run(req, variant, poke=None):
@@ -81,12 +82,12 @@ def test_a_mostly_dry_reach_skips_tiles_and_keeps_every_bit(pool):
 
     req = reach_request(0, 0.0, pool, np.random.default_rng(4), end_time=1800.0, scale=0.25)
     req.interval_s = 600.0  # two morphology stops inside the run
-    tile, strips = _families()
-    want = _run(req, tile)
+    ref, skipping = _families()
+    want = _run(req, ref)
     assert (want["h"] == 0).mean() > 0.6  # most of the floodplain never wets
     assert not np.array_equal(want["z"], req.bed)  # the stops moved the bed
-    for v in strips:
-        _same(want, _run(req, v), f"strip variant {v}")
+    for v in skipping:
+        _same(want, _run(req, v), f"variant {v}")
 
 
 def test_a_basin_drying_after_its_rain_keeps_every_bit(pool):
@@ -95,11 +96,11 @@ def test_a_basin_drying_after_its_rain_keeps_every_bit(pool):
     req = basin_request(0, 0.0, pool, np.random.default_rng(6), n_min=150, n_max=200,
                         end_time=3600.0)
     req.infil = req.infil * np.float32(20.0)  # strong enough to dry hillslopes to 0
-    tile, strips = _families()
-    want = _run(req, tile)
+    ref, skipping = _families()
+    want = _run(req, ref)
     assert (want["h"] == 0).any()
-    for v in strips:
-        _same(want, _run(req, v), f"strip variant {v}")
+    for v in skipping:
+        _same(want, _run(req, v), f"variant {v}")
 
 
 def test_uploads_and_tiling_changes_in_mid_run_clear_the_flags(pool):
@@ -111,7 +112,7 @@ def test_uploads_and_tiling_changes_in_mid_run_clear_the_flags(pool):
 
     req = reach_request(1, 0.0, pool, np.random.default_rng(8), end_time=1800.0, scale=0.25)
     req.interval_s = 600.0
-    tile, strips = _families()
+    ref, skipping = _families()
     ny, nx = req.bed.shape
 
     def poke(switch_to):
@@ -125,7 +126,7 @@ def test_uploads_and_tiling_changes_in_mid_run_clear_the_flags(pool):
                 _lib.call("rbs_set_li_variant", st._handle, ctypes.c_int32(switch_to))
         return f
 
-    want = _run(req, tile, poke(None))
+    want = _run(req, ref, poke(None))
     assert want["h"][:, :10].any()  # the poured water is still on the floodplain
-    for a, b in zip(strips, strips[1:] + strips[:1]):
-        _same(want, _run(req, a, poke(b)), f"strip variant {a} -> {b}")
+    for a, b in zip(skipping, skipping[1:] + skipping[:1]):
+        _same(want, _run(req, a, poke(b)), f"variant {a} -> {b}")
