@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--workload", default="li",
                     choices=["li", "ensemble", "serve", "hllc", "copy", "channels", "output",
                              "reservoir", "reach"])
+    ap.add_argument("--host-loop", action="store_true",
+                    help="reach: drive the runs from the host tick loop (no graphs)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time every LI kernel variant on the spun-up state")
     ap.add_argument("--scenarios", type=int, default=256, help="ensemble size")
@@ -742,10 +744,22 @@ def bench_reach(args) -> int:
         st.sync()
         t0 = time.perf_counter()
         steps = 0
-        for seg in run.sched.segments():
-            steps += run.run_until(seg.t_stop)
-            for proc, el in seg.due:
-                proc.advance(seg.t_stop, el)
+        if args.host_loop:  # the reference's tick loop: one step launch per tick
+            from rbs.core import local_inertial as li
+            from rbs.processes.inflow import InflowInjector
+
+            inj = InflowInjector(req.inflows, st)
+            for tick in run.sched.ticks(lambda: li.compute_dt(st, 0.7, 30.0)):
+                inj.apply(st, tick.t0, tick.dt)
+                li.step(st, tick.dt, rain=0.0, rain_scale=0.0, t=tick.t0)
+                steps += 1
+                for proc, el in tick.due:
+                    proc.advance(tick.t1, el)
+        else:
+            for seg in run.sched.segments():
+                steps += run.run_until(seg.t_stop)
+                for proc, el in seg.due:
+                    proc.advance(seg.t_stop, el)
         st.sync()
         wall = time.perf_counter() - t0
         h = st.h.numpy()
@@ -776,6 +790,7 @@ def bench_reach(args) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"li_exner_reach_{ny}x{nx}", "grid": f"{ny}x{nx}", "dx_m": 2.0,
+                   "driver": "host tick loop" if args.host_loop else "device-driven graphs",
                    "sim_s": 1800.0, "sources": "point inflow hydrograph, bedload integral, "
                    "morphology every 900 s, open south edge"},
         "kernel": best,
