@@ -1034,6 +1034,9 @@ static const LiVariant kLiVariants[] = {
     RBS_LI_TILE(128, 8),     RBS_LI_TILE(128, 16),    RBS_LI_TILE(32, 32),
     RBS_LI_STRIP(16, 6),     RBS_LI_STRIP(32, 6),     RBS_LI_STRIP(64, 6),
     RBS_LI_ROWS(16, 6, 8, 2), RBS_LI_ROWS(32, 6, 8, 2), RBS_LI_ROWS(50, 6, 8, 2),
+    // 8-row strips: a short serial chain per wave for grids where a few wet tiles
+    // among skipped dry ones set the step's time
+    RBS_LI_STRIP(8, 6),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;    // strips of 32 rows
@@ -1335,9 +1338,11 @@ int li_default_variant() { return kDefaultLiVariant; }
 int li_initial_variant(int nx) { return nx >= 2048 ? kRowsVariantForWide : kDefaultLiVariant; }
 int li_wide_batch_variant() { return kRowsVariantForWide; }
 int li_wide_batch_min_nx() { return 2048; }
-// batches of bedload members: strips of 16 rows (measured fastest inside a batch,
-// where the members' tiles fill the GPU whatever their shapes)
-int li_default_sed_variant() { return 6; }
+// batches of bedload members: strips of 8 rows.  Strips beat the LDS tiles inside a
+// batch, where the members' tiles fill the GPU whatever their shapes; the bedload
+// members are alluvial reaches, mostly dry floodplain, whose few wet tiles set the
+// step's time, and 8 rows halve a wet tile's serial chain against 16.
+int li_default_sed_variant() { return 12; }
 void li_variant_shape(int v, int *bx, int *by, int *family) {
   const LiVariant &x = variant_at(v);
   *bx = x.bx;
