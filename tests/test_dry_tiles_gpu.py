@@ -130,3 +130,50 @@ def test_uploads_and_tiling_changes_in_mid_run_clear_the_flags(pool):
     assert want["h"][:, :10].any()  # the poured water is still on the floodplain
     for a, b in zip(skipping, skipping[1:] + skipping[:1]):
         _same(want, _run(req, a, poke(b)), f"variant {a} -> {b}")
+
+
+def test_a_state_moved_between_kernels_by_a_batch_keeps_every_bit(pool):
+    """A reach stepped alone on the 64 x 8 LDS tile (its flags set for that tiling),
+    then in a batch, which moves it to the bedload strips and clears its flags in the
+    batch's stream order, then alone again on the batch's kernel: the bits of a
+    reach that stops at the same times on the row blocks."""
+    from rbs import _lib
+    from rbs.ensemble import Batch
+    from rbs.serve import build_state, reach_request
+
+    req = reach_request(2, 0.0, pool, np.random.default_rng(10), end_time=1800.0, scale=0.25)
+    req.interval_s = 1.0e6  # the bedload integral without activations
+    ref, _ = _families()
+
+    def states(variant):
+        st, run, _ = build_state(req, tune_kernel=False)
+        _lib.call("rbs_set_li_variant", st._handle, ctypes.c_int32(variant))
+        return st, run
+
+    st_ref, run_ref = states(ref)
+    for t in (600.0, 1200.0, 1800.0):
+        run_ref.run_until(t)
+    st, run = states(_tile_variants()[0])
+    run.run_until(600.0)
+    b = Batch(4)
+    slot = b.add(run, 1200.0)
+    b.run_to_completion(16)
+    b.remove(slot)
+    b.close()
+    run.run_until(1800.0)
+    for name in ("h", "qx", "qy"):
+        assert np.array_equal(getattr(st, name).numpy(), getattr(st_ref, name).numpy()), name
+    for name in ("qs_int_x", "qs_comp_x", "qs_int_y", "qs_comp_y"):
+        assert np.array_equal(getattr(st.sediment, name).numpy(),
+                              getattr(st_ref.sediment, name).numpy()), name
+    assert st.ledger().steps == st_ref.ledger().steps and st.ledger().t == st_ref.ledger().t
+    assert (st.h.numpy() == 0).mean() > 0.6
+    st.close()
+    st_ref.close()
+
+
+def _tile_variants():
+    from rbs.autotune import variant_table
+
+    return [r["variant"] for r in variant_table()
+            if r["family"] == "lds_tile" and r["tile"] == (64, 8)]
