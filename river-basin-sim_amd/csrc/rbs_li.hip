@@ -971,7 +971,8 @@ __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_kernel(LiArgs a)
 }
 
 // A batch of states in one launch per tile segment (seg 0: members without the
-// bedload integral, seg 1: with it): the segment's flat tile space, mapped back to
+// bedload integral, seg 1: with it, seg 2: wide members without it on the row
+// blocks): the segment's flat tile space, mapped back to
 // (member, tile) by a table; each member keeps its own clock, dt and ledger.  The
 // grid is sized for the batch's capacity; blocks past the segment's live tile count
 // return at once.
