@@ -97,12 +97,18 @@ def lib() -> ctypes.CDLL:
             getattr(L, fn).restype = ctypes.c_float
         L.orc_manning_denominator_radius.argtypes = [ctypes.c_float] * 6
         L.orc_manning_denominator_radius.restype = ctypes.c_float
-        for fn in ("orc_pow73", "orc_rpow73", "orc_pow43", "orc_cbrt", "orc_pow15"):
+        for fn in ("orc_pow73", "orc_pow43", "orc_cbrt", "orc_pow15"):
             getattr(L, fn).argtypes = [ctypes.c_float]
             getattr(L, fn).restype = ctypes.c_float
+        L.orc_ln_coarse.argtypes = [ctypes.c_float]
+        L.orc_ln_coarse.restype = ctypes.c_double
         L.orc_set_pow_exact.argtypes = [ctypes.c_int]
-        L.orc_pow_sweep.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_get_pow_exact.restype = ctypes.c_int
+        L.orc_pow_sweep.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
         L.orc_pow_sweep.restype = ctypes.c_long
+        L.orc_pow_misses.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_long]
+        L.orc_pow_misses.restype = ctypes.c_long
         L.orc_fill_span.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_fill_span.restype = ctypes.c_double
         _lib = L
@@ -114,10 +120,31 @@ def set_threads(n: int) -> None:
 
 
 def set_pow(mode: str) -> None:
-    """'exact': correctly rounded powers (the GPU's, default); 'libm': glibc powf."""
+    """'exact': the correctly rounded powers with the reference's float32 exponents
+    (the GPU's sequence, the default); 'libm': glibc powf with the same exponents."""
     if mode not in ("exact", "libm"):
         raise ValueError(mode)
     lib().orc_set_pow_exact(1 if mode == "exact" else 0)
+
+
+def get_pow() -> str:
+    return "exact" if lib().orc_get_pow_exact() else "libm"
+
+
+class pow_mode:
+    """Context manager: run the oracle with the given power mode, then restore."""
+
+    def __init__(self, mode: str):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = get_pow()
+        set_pow(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        set_pow(self.prev)
+        return False
 
 
 def _ptr(a: np.ndarray | None, kind=_fp):
@@ -341,13 +368,23 @@ def pow73(x: float) -> float:
     return float(lib().orc_pow73(x))
 
 
-def rpow73(x: float) -> float:
-    return float(lib().orc_rpow73(x))
+def ln_coarse(x: float) -> float:
+    return float(lib().orc_ln_coarse(x))
 
 
-def pow_sweep(kind: int, lo: int, hi: int) -> int:
-    """Float32 bit patterns in [lo, hi) whose power differs from powl's, rounded."""
-    return int(lib().orc_pow_sweep(kind, lo, hi))
+def pow_sweep(kind: int, lo: int, hi: int, libm: bool = False) -> int:
+    """Float32 bit patterns in [lo, hi) whose power (kind 0..3: x^f32(7/3), x^f32(4/3),
+    x^f32(1/3), x^1.5) differs from powl's with the same exponent, rounded; libm
+    counts glibc powf's instead of the oracle's sequence."""
+    return int(lib().orc_pow_sweep(kind, lo, hi, 1 if libm else 0))
+
+
+def pow_misses(kind: int, lo: int, hi: int, maxn: int = 64) -> np.ndarray:
+    """The float32 inputs in [lo, hi) (bit patterns) where the oracle's power differs
+    from powl's, rounded (kind 0..2), at most maxn of them."""
+    out = np.zeros(maxn, dtype=np.uint32)
+    n = lib().orc_pow_misses(kind, lo, hi, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), maxn)
+    return out[:n].view(np.float32).copy()
 
 
 def pow43(x: float) -> float:

@@ -45,17 +45,24 @@ static inline float fmn(float a, float b) { return a < b ? a : b; }
 static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo), hi); }
 
 /*
- * Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
- * Warp's wp.pow is a libm powf, accurate to about an ulp, whose last bit differs
- * between math libraries.  Both this oracle and the HIP kernels evaluate the powers
- * the solver needs by the same sequence of IEEE basic operations (fused multiply-adds
- * only where written as fmaf/fma, which are correctly rounded), so the two agree bit
- * for bit, and the result is the correctly rounded float32 power except in vanishingly
- * rare ties: x^(-1/3) from a bit-pattern seed refined by two float32 Newton steps
- * (relative error below 2e-5), then one float64 step of third order,
- * b (1 + r/3 + 2r^2/9 + 14r^3/81) with r = 1 - x b^3 formed exactly by an fma, and
- * the power formed in float64 and rounded once.  tests/test_oracle_pow.py checks each
- * against the exact power, every float32 of three binades and random ones elsewhere.
+ * Powers of a positive float32 (wp.pow in friction.py:31, :57, sediment.py:168, :192).
+ * The reference's exponents are Python floats handed to a float32 pow, so they are
+ * float32(7/3), float32(4/3), float32(1/3) and 1.5.  Warp's wp.pow is a libm powf,
+ * accurate to about an ulp, whose last bit differs between math libraries.
+ *
+ * Mode 1 (the default) is the correctly rounded float32 power x^e, e = float32(p/3) =
+ * p/3 - d, by a fixed sequence of IEEE basic operations (fused multiply-adds only where
+ * written as fmaf/fma, which are correctly rounded), so it is the same bits on every
+ * IEEE machine; the HIP kernels evaluate the same sequence:
+ *   x^(-1/3): a bit-pattern seed, two float32 Newton steps (relative error < 2e-5),
+ *     one float64 step of third order, b (1 + r/3 + 2r^2/9 + 14r^3/81), r = 1 - x b^3
+ *     formed exactly by an fma; x^(p/3) formed from it in float64;
+ *   x^(-d) = exp(-d ln x) = 1 - u + u^2/2, u = d ln x, with ln x from a float32
+ *     polynomial (3e-8 absolute, so u is good to 2.4e-15 relative);
+ *   the product rounded once.
+ * Mode 0 calls the C library's powf with the float32 exponents, as the reference's
+ * Warp-CPU runs did; its published round-off-level mass residuals reproduce with it.
+ * tests/test_oracle_pow.py checks mode 1 against the exact powers and against powl.
  */
 static inline double inv_cbrt64(float x) {
   uint32_t i;
@@ -73,66 +80,125 @@ static inline double inv_cbrt64(float x) {
   double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
   return fma(b * r, c, b);
 }
-/* 1: the correctly rounded powers above (the GPU's); 0: the C library's powf, which
- * is what the reference's Warp-CPU runs evaluated (its published round-off-level
- * mass residuals reproduce only with it). */
+/* ln x, x = 2^k m normal with m in [sqrt(1/2), sqrt(2)), k shifted by kadj:
+ * ln(1 + t) = t - t^2/2 + t^3 R(t), t = m - 1 (exact), R of degree 6 fitted to the
+ * remainder on that range; float32, then k ln 2 in float64.  3e-8 absolute. */
+static inline double ln_coarse(float x, int kadj) {
+  uint32_t i;
+  memcpy(&i, &x, 4);
+  int k = (int)(i >> 23) - 127 + kadj;
+  uint32_t mb = (i & 0x007fffffu) | 0x3f800000u;
+  if (mb > 0x3fb504f3u) {
+    mb -= 0x00800000u;
+    k += 1;
+  }
+  float m;
+  memcpy(&m, &mb, 4);
+  float t = m - 1.0f;
+  float r = 0.08672088310927706f;
+  r = fmaf(r, t, -0.14379001534630725f);
+  r = fmaf(r, t, 0.1497802208269855f);
+  r = fmaf(r, t, -0.16564426496452503f);
+  r = fmaf(r, t, 0.19954873155337902f);
+  r = fmaf(r, t, -0.2500169873952298f);
+  r = fmaf(r, t, 0.3333416597960291f);
+  float s2 = fmaf(r, t, -0.5f);
+  float l = fmaf(t * t, s2, t);
+  return fma((double)k, 0.6931471805599453, (double)l);
+}
+/* x^e, e = float32(p/3), p in {1, 4, 7}, x >= 0; subnormal x is first scaled by 2^48
+ * (a power of two divisible by three) and the power scaled back by 2^(-16p). */
+static float pow_third(float x, int p) {
+  const double d = ((double)p - 3.0 * (double)(float)((double)p / 3.0)) / 3.0;
+  float xs = x;
+  int kadj = 0, tiny = x < 0x1p-126f;
+  if (tiny) {
+    xs = x * 0x1p48f;
+    kadj = -48;
+  }
+  double a = (double)xs, b = inv_cbrt64(xs), pw;
+  if (p == 7)
+    pw = ((a * a) * a) * (b * b);
+  else if (p == 4)
+    pw = (a * a) * (b * b);
+  else
+    pw = a * (b * b);
+  if (tiny) pw *= p == 7 ? 0x1p-112 : p == 4 ? 0x1p-64 : 0x1p-16;
+  double u = d * ln_coarse(xs, kadj);
+  double c = fma(u, fma(0.5, u, -1.0), 1.0);
+  return (float)(pw * c);
+}
 static int g_pow_exact = 1;
 void orc_set_pow_exact(int on) { g_pow_exact = on != 0; }
-float orc_pow73(float x) { /* x^(7/3) = x^3 * x^(-2/3) */
+int orc_get_pow_exact(void) { return g_pow_exact; }
+float orc_pow73(float x) { /* x^float32(7/3) */
   if (!g_pow_exact) return powf(x, (float)(7.0 / 3.0));
-  double a = (double)x, b = inv_cbrt64(x);
-  return (float)(((a * a) * a) * (b * b));
+  return pow_third(x, 7);
 }
-float orc_rpow73(float x) { /* x^(-7/3) = (x^(-1/3))^7 */
-  double b = inv_cbrt64(x), b2 = b * b;
-  return (float)(((b2 * b2) * b2) * b);
-}
-float orc_cbrt(float x) { /* x^(1/3) = x * x^(-2/3) */
+float orc_cbrt(float x) { /* x^float32(1/3) */
   if (!g_pow_exact) return powf(x, (float)(1.0 / 3.0));
-  double a = (double)x, b = inv_cbrt64(x);
-  return (float)(a * (b * b));
+  return pow_third(x, 1);
 }
-float orc_pow43(float x) { /* x^(4/3) = x^2 * x^(-2/3) */
+float orc_pow43(float x) { /* x^float32(4/3) */
   if (!g_pow_exact) return powf(x, (float)(4.0 / 3.0));
-  double a = (double)x, b = inv_cbrt64(x);
-  return (float)((a * a) * (b * b));
+  return pow_third(x, 4);
 }
 float orc_pow15(float x) { /* x^(3/2) = x * sqrt(x) */
   if (!g_pow_exact) return powf(x, 1.5f);
   double a = (double)x;
   return (float)(a * sqrt(a));
 }
+double orc_ln_coarse(float x) { return ln_coarse(x, 0); }
 
-/* friction.py:104-117 manning_denominator, 1 + g dt n^2 |q| / h^(7/3).  The exact-power
- * mode takes the quotient as a product with the correctly rounded h^(-7/3) (one
- * rounding in place of two, no float32 division), as the HIP kernels do; the libm mode
- * keeps the reference's division by powf. */
+/* friction.py:18-31 manning_denominator, 1 + g dt n^2 |q| / h^float32(7/3): the
+ * product left to right, one float32 division, one sum (both modes). */
 static inline float manning_denominator(float q_prev, float h_flow, float n, float g,
                                         float dt) {
-  if (!g_pow_exact) return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
-  return 1.0f + g * dt * n * n * fabsf(q_prev) * orc_rpow73(h_flow);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / orc_pow73(h_flow);
 }
 
-/* Every float32 in [lo, hi) (bit patterns) checked against powl rounded to float32;
- * returns the number that differ.  kind 0..4: x^(7/3), x^(-7/3), x^(1/3), x^(4/3),
- * x^(3/2).  Test support only. */
-long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi) {
-  static const long double ex[5] = {7.0L / 3.0L, -7.0L / 3.0L, 1.0L / 3.0L, 4.0L / 3.0L, 1.5L};
+/* Every float32 in [lo, hi) (bit patterns) checked against powl with the same float32
+ * exponent, rounded to float32; returns the number that differ.  kind 0..3:
+ * x^float32(7/3), x^float32(4/3), x^float32(1/3), x^(3/2).  libm != 0 checks the C
+ * library's powf instead of mode 1.  Test support only. */
+long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi, int libm) {
+  const float ex[4] = {(float)(7.0 / 3.0), (float)(4.0 / 3.0), (float)(1.0 / 3.0), 1.5f};
   long bad = 0;
 #pragma omp parallel for reduction(+ : bad) schedule(static)
   for (long long u = (long long)lo; u < (long long)hi; ++u) {
     uint32_t bits = (uint32_t)u;
     float x;
     memcpy(&x, &bits, 4);
-    float want = (float)powl((long double)x, ex[kind]);
-    float got = kind == 0 ? orc_pow73(x) : kind == 1 ? orc_rpow73(x) : kind == 2 ? orc_cbrt(x)
-              : kind == 3 ? orc_pow43(x) : orc_pow15(x);
+    float want = (float)powl((long double)x, (long double)ex[kind]);
+    float got;
+    if (libm)
+      got = powf(x, ex[kind]);
+    else if (kind == 3) {
+      double a = (double)x;
+      got = (float)(a * sqrt(a));
+    } else
+      got = pow_third(x, kind == 0 ? 7 : kind == 1 ? 4 : 1);
     bad += got != want;
   }
   return bad;
 }
 
-/* friction.py:35-57 manning_denominator_radius: D = 1 + g dt n^2 |q| / (h R^(4/3)) */
+/* The first maxn bit patterns in [lo, hi) that orc_pow_sweep counts (mode 1), written
+ * to out in increasing order; returns how many were written.  Test support only. */
+long orc_pow_misses(int kind, uint32_t lo, uint32_t hi, uint32_t *out, long maxn) {
+  const float ex[3] = {(float)(7.0 / 3.0), (float)(4.0 / 3.0), (float)(1.0 / 3.0)};
+  long n = 0;
+  for (long long u = (long long)lo; u < (long long)hi && n < maxn; ++u) {
+    uint32_t bits = (uint32_t)u;
+    float x;
+    memcpy(&x, &bits, 4);
+    float want = (float)powl((long double)x, (long double)ex[kind]);
+    if (pow_third(x, kind == 0 ? 7 : kind == 1 ? 4 : 1) != want) out[n++] = bits;
+  }
+  return n;
+}
+
+/* friction.py:35-57 manning_denominator_radius: D = 1 + g dt n^2 |q| / (h R^float32(4/3)) */
 float orc_manning_denominator_radius(float q_prev, float h_flow, float radius, float n,
                                      float g, float dt) {
   return 1.0f + g * dt * n * n * fabsf(q_prev) / (h_flow * orc_pow43(radius));
@@ -181,7 +247,7 @@ static inline float channel_flux(float q_prev, float eta_max, float slope, float
 static inline float floodplain_flux(float q_prev, float eta_max, float slope, float z_bank,
                                     float n_fp, float dt, float g);
 
-/* sources.py:274-290 kahan_add -> (total, comp) */
+/* sources.py:66-82 kahan_add -> (total, comp) */
 static inline void kahan_add(float *total, float *comp, float inc) {
   float y = inc - *comp;
   float t = *total + y;
@@ -431,14 +497,14 @@ static void apply_rain_field_plain(orc_state *s, float dt, float scale) {
   for (size_t k = 0; k < cells; ++k) s->h[k] = s->h[k] + s->rain[k] * scale * dt;
 }
 
-/* sources.py:293-309 add_uniform_rain_c */
+/* sources.py:85-101 add_uniform_rain_c */
 static void add_uniform_rain_c(orc_state *s, float rate, float dt) {
   const size_t cells = (size_t)s->ny * s->nx;
 #pragma omp parallel for schedule(static)
   for (size_t k = 0; k < cells; ++k) kahan_add(&s->h[k], &s->h_comp[k], rate * dt);
 }
 
-/* sources.py:312-329 add_rain_field_c */
+/* sources.py:104-121 add_rain_field_c */
 static void add_rain_field_c(orc_state *s, float dt, float scale) {
   const size_t cells = (size_t)s->ny * s->nx;
 #pragma omp parallel for schedule(static)

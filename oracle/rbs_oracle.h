@@ -55,19 +55,24 @@ typedef struct orc_state {
   float *qy_ch, *qy_fp;           /* (ny+1, nx) */
 } orc_state;
 
-/* The powers the solver takes (see rbs_oracle.c): x^(7/3), x^(-7/3), x^(4/3), x^(1/3),
- * x^(3/2). */
+/* The powers the solver takes (see rbs_oracle.c): x^float32(7/3), x^float32(4/3),
+ * x^float32(1/3), x^(3/2). */
 float orc_pow73(float x);
-float orc_rpow73(float x);
 float orc_pow43(float x);
 float orc_cbrt(float x);
 float orc_pow15(float x);
+/* The float32 polynomial ln x inside the powers (test support). */
+double orc_ln_coarse(float x);
 /* 1 (default): correctly rounded powers, bit-identical to the HIP kernels;
  * 0: the C library's powf, as the reference's Warp-CPU runs evaluated them. */
 void orc_set_pow_exact(int on);
-/* Test support: how many float32 bit patterns in [lo, hi) give a power (kind 0..4:
- * x^(7/3), x^(-7/3), x^(1/3), x^(4/3), x^(3/2)) other than powl's, rounded. */
-long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi);
+int orc_get_pow_exact(void);
+/* Test support: how many float32 bit patterns in [lo, hi) give a power (kind 0..3:
+ * x^float32(7/3), x^float32(4/3), x^float32(1/3), x^(3/2)) other than powl's with the
+ * same exponent, rounded; libm != 0 counts the C library's powf instead. */
+long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi, int libm);
+/* The first maxn of those bit patterns (mode 1, kind 0..2); returns how many. */
+long orc_pow_misses(int kind, uint32_t lo, uint32_t hi, uint32_t *out, long maxn);
 
 /* Threads used by every pass (OpenMP); 1 gives the serial order of Warp-CPU. */
 void orc_set_threads(int n);

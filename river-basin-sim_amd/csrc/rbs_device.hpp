@@ -26,16 +26,24 @@ __device__ __forceinline__ gptr<T> glob(T *p) {
   return (gptr<T>)p;
 }
 
-// Powers of a positive float32 (wp.pow in friction.py:57, :117, sediment.py:168, :192).
-// The device library's powf and the host's differ in the last bit, and at wet/dry
-// switches that difference grows; so the solver evaluates its powers by one fixed
-// sequence of IEEE basic operations (the build sets -ffp-contract=off, so the only
-// fused multiply-adds are the fmaf/fma written here, correctly rounded everywhere),
-// bit-identical on every IEEE machine and correctly rounded but for vanishingly rare
-// ties: x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative error
-// below 2e-5), then one float64 step of third order -- b (1 + r/3 + 2r^2/9 + 14r^3/81)
-// with r = 1 - x b^3 formed exactly by an fma -- that leaves it within a few float64
-// ulps; the power is formed in float64 and rounded once.
+// Powers of a positive float32 (wp.pow in friction.py:31, :57, sediment.py:168, :192).
+// The reference passes the exponents as Python floats into a float32 pow, so they are
+// float32(7/3) = 7/3 - 7.9e-8, float32(4/3) = 4/3 + 4.0e-8 and float32(1/3) = 1/3 +
+// 9.9e-9, and 1.5 exactly.  A libm powf is good to about an ulp and its last bit
+// differs between math libraries; at wet/dry switches that bit grows.  So the solver
+// evaluates x^e, e = float32(p/3) = p/3 - d, as the correctly rounded float32 power,
+// by one fixed sequence of IEEE basic operations (the build sets -ffp-contract=off, so
+// the only fused multiply-adds are the fmaf/fma written here, correctly rounded
+// everywhere), bit-identical on every IEEE machine:
+//  1. x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative error
+//     below 2e-5), then one float64 step of third order -- b (1 + r/3 + 2r^2/9 +
+//     14r^3/81) with r = 1 - x b^3 formed exactly by an fma -- within a few float64
+//     ulps; x^(p/3) formed from it in float64;
+//  2. the factor x^(-d) = exp(-d ln x) = 1 - u + u^2/2, u = d ln x (|u| < 1e-5 over
+//     the float32 range, so the next term is below 2e-16), with ln x to 3e-8 absolute
+//     (below): u is then good to 2.4e-15 relative;
+//  3. the product rounded once to float32.
+// Float64 here buys the last float32 bit of a power; every field stays float32.
 __device__ __forceinline__ double inv_cbrt64(float x) {
   const uint32_t j = 0x54a2fa8cu - __float_as_uint(x) / 3u;
   float y = __uint_as_float(j);
@@ -52,35 +60,75 @@ __device__ __forceinline__ double inv_cbrt64(float x) {
   const double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
   return fma(b * r, c, b);
 }
-__device__ __forceinline__ float pow73(float x) {  // x^(7/3) = x^3 * x^(-2/3)
-  const double a = (double)x, b = inv_cbrt64(x);
-  return (float)(((a * a) * a) * (b * b));
+// ln x for a positive normal float32 x = 2^k m, m in [sqrt(1/2), sqrt(2)) (k adjusted
+// by kadj), to 3e-8 absolute: ln(1 + t) = t - t^2/2 + t^3 R(t), t = m - 1 exact, with
+// R a degree-6 polynomial fitted to the remainder on that range (error 7e-9), in
+// float32; k ln 2 added in float64.
+__device__ __forceinline__ double ln_coarse(float x, int kadj) {
+  const uint32_t i = __float_as_uint(x);
+  int k = (int)(i >> 23) - 127 + kadj;
+  uint32_t mb = (i & 0x007fffffu) | 0x3f800000u;
+  if (mb > 0x3fb504f3u) {  // m > sqrt(2): use m / 2
+    mb -= 0x00800000u;
+    k += 1;
+  }
+  const float t = __uint_as_float(mb) - 1.0f;
+  float r = 0.08672088310927706f;
+  r = fmaf(r, t, -0.14379001534630725f);
+  r = fmaf(r, t, 0.1497802208269855f);
+  r = fmaf(r, t, -0.16564426496452503f);
+  r = fmaf(r, t, 0.19954873155337902f);
+  r = fmaf(r, t, -0.2500169873952298f);
+  r = fmaf(r, t, 0.3333416597960291f);
+  const float s = fmaf(r, t, -0.5f);
+  const float l = fmaf(t * t, s, t);
+  return fma((double)k, 0.6931471805599453, (double)l);
 }
-__device__ __forceinline__ float rpow73(float x) {  // x^(-7/3) = (x^(-1/3))^7
-  const double b = inv_cbrt64(x), b2 = b * b;
-  return (float)(((b2 * b2) * b2) * b);
+
+// x^e for e = float32(P/3), P in {1, 4, 7}, x >= 0.  SUB admits subnormal x (scaled
+// by 2^48, a power of 2 divisible by 3, first); without it x must be 0 or normal.
+template <int P, bool SUB>
+__device__ __forceinline__ float pow_third(float x) {
+  // d = P/3 - float32(P/3); P - 3 float32(P/3) is exact in float64
+  constexpr double d = ((double)P - 3.0 * (double)(float)((double)P / 3.0)) / 3.0;
+  float xs = x;
+  int kadj = 0;
+  bool tiny = false;
+  if (SUB) {
+    tiny = x < 0x1p-126f;
+    if (tiny) {
+      xs = x * 0x1p48f;
+      kadj = -48;
+    }
+  }
+  const double a = (double)xs, b = inv_cbrt64(xs);
+  double p;
+  if (P == 7) p = ((a * a) * a) * (b * b);
+  else if (P == 4) p = (a * a) * (b * b);
+  else p = a * (b * b);
+  if (SUB && tiny) p *= (P == 7 ? 0x1p-112 : P == 4 ? 0x1p-64 : 0x1p-16);
+  const double u = d * ln_coarse(xs, kadj);
+  const double c = fma(u, fma(0.5, u, -1.0), 1.0);
+  return (float)(p * c);
 }
-__device__ __forceinline__ float cbrt_f(float x) {  // x^(1/3) = x * x^(-2/3)
-  const double a = (double)x, b = inv_cbrt64(x);
-  return (float)(a * (b * b));
-}
-__device__ __forceinline__ float pow43(float x) {  // x^(4/3) = x^2 * x^(-2/3)
-  const double a = (double)x, b = inv_cbrt64(x);
-  return (float)((a * a) * (b * b));
-}
+// wp.pow(h, 7.0/3.0) for a depth h >= H_DRY (friction.py:31)
+__device__ __forceinline__ float pow_e73(float x) { return pow_third<7, false>(x); }
+// wp.pow(R, 4.0/3.0) and wp.pow(R, 1.0/3.0) of a hydraulic radius (friction.py:57,
+// sediment.py:168), which a narrow channel can make arbitrarily small
+__device__ __forceinline__ float pow_e43(float x) { return pow_third<4, true>(x); }
+__device__ __forceinline__ float pow_e13(float x) { return pow_third<1, true>(x); }
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   const double a = (double)x;
   return (float)(a * sqrt(a));
 }
 
-// friction.py:104-117 manning_denominator, 1 + g dt n^2 |q| / h^(7/3).  The quotient
-// by the power is taken as a product with the correctly rounded h^(-7/3): one
-// rounding in place of the reference's two, and no float32 division.
+// friction.py:18-31 manning_denominator, 1 + g dt n^2 |q| / h^e, e = float32(7/3), in
+// the reference's float32 order: the product left to right, one division, one sum.
 __device__ __forceinline__ float manning_den(float q_prev, float h_flow, float n, float g, float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) * rpow73(h_flow);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / pow_e73(h_flow);
 }
 
-// sources.py:274-290 kahan_add
+// sources.py:66-82 kahan_add
 __device__ __forceinline__ void kahan_add(float &total, float &comp, float inc) {
   float y = inc - comp;
   float t = total + y;
@@ -189,7 +237,7 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
       const float radius = w_face * h_ch / (w_face + 2.0f * h_ch);
       const float nc = 0.5f * (a.nc + b.nc);
       const float num = q_ch - g * h_ch * dt * slope;
-      const float den = 1.0f + g * dt * nc * nc * fabsf(q_ch) / (h_ch * pow43(radius));
+      const float den = 1.0f + g * dt * nc * nc * fabsf(q_ch) / (h_ch * pow_e43(radius));
       ch = num / den;
     }
   }
@@ -223,7 +271,7 @@ __device__ __forceinline__ float drain(float h_edge, float q_out, float dxdt) {
 __device__ __forceinline__ float face_capacity(float q, float h, float radius, float n, float d50) {
   const float sg = (float)1.65, th_c = (float)0.047;
   if (h < kHDry || d50 <= 0.0f || q == 0.0f) return 0.0f;
-  float shear = kG * n * n * q * q / (h * h * cbrt_f(radius));
+  float shear = kG * n * n * q * q / (h * h * pow_e13(radius));
   float theta = shear / (sg * kG * d50);
   if (theta <= th_c) return 0.0f;
   float excess = theta - th_c;

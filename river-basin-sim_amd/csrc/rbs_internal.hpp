@@ -25,8 +25,6 @@ constexpr double kGravity = 9.81;
 constexpr double kHDry64 = 1.0e-3;
 // solver/scheduler.py:45
 constexpr double kEpsT = 1e-6;
-// friction.py:117: Python folds 7.0/3.0 in float64; the kernel parameter is float32.
-constexpr float kSevenThirds = (float)(7.0 / 3.0);
 
 constexpr int kThreads = 256;  // block size of every tiled kernel
 

@@ -3,12 +3,12 @@
 // One launch performs the whole of local_inertial.step (local_inertial.py:547-727)
 // for a tile of BY x BX cells:
 //   eta = h + z                                    (:61-69)
-//   interior face update + dry guard + friction    (:72-129, friction.py:104-117)
+//   interior face update + dry guard + friction    (:72-129, friction.py:18-31)
 //   closed edge faces = 0                          (boundaries.py:73-91)
 //   donor outflow limiter beta, limit faces        (:197-263)
 //   bedload transport integral (sediment armed)    (sediment.py:218-273)
 //   continuity + uniform rain                      (:132-151)
-//   Kahan areal sources / plain rain field         (sources.py:293-329, :154-167)
+//   Kahan areal sources / plain rain field         (sources.py:85-121, :154-167)
 //   capped infiltration                            (:170-194)
 //   open-edge drains, west/east/north/south order  (boundaries.py:109-208)
 // The reference's separate launches become phases separated by __syncthreads over
@@ -119,7 +119,7 @@ __device__ __forceinline__ StepScalars step_scalars(const LiArgs &a, CtlPtr ctl)
   return k;
 }
 
-// update_h (:132-151), areal sources (sources.py:293-329, local_inertial.py:154-167),
+// update_h (:132-151), areal sources (sources.py:85-121, local_inertial.py:154-167),
 // capped infiltration (:170-194) and the open-edge drains in the reference's launch
 // order (boundaries.py:181-208) for cell (r, c) with its limited faces.  `comp`
 // (Kahan debt), `rain_v` and `infil_v` are that cell's values, read by the caller;

@@ -1,7 +1,9 @@
 """HLLC HIP path (two fused SSP-RK2 stage kernels) against the oracle.
 
 Same tolerance as the local-inertial path: max|dh| and max|d(hu, hv)| <= 1e-5
-after the stated number of steps with identical dt sequences.
+after the stated number of steps with identical dt sequences, against the oracle's
+default mode (the correctly rounded float32-exponent powers the kernels evaluate)
+and its libm mode (glibc powf, as the reference's Warp-CPU runs evaluated).
 """
 
 from __future__ import annotations
@@ -74,7 +76,8 @@ def test_open_and_stage_edges_rain_infiltration(rbs, orc):
     curve = [(0.0, 0.5), (200.0, 0.9), (400.0, 0.4)]
     g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.035)
     o = orc.State(z, 5.0, depth=d0, manning=0.035)
-    for s in (g, o):
+    o_libm = orc.State(z, 5.0, depth=d0, manning=0.035)
+    for s in (g, o, o_libm):
         s.set_open_boundaries({"west": "open", "north": "open", "east": "fixed_stage"},
                               {"east": curve})
         s.set_rain_field(rain)
@@ -88,16 +91,47 @@ def test_open_and_stage_edges_rain_infiltration(rbs, orc):
         scale = 1.0 if k < 200 else 0.0
         rbs.hl.step(g, dt, rain_scale=scale, t=t)
         orc.hllc_step(o, dt, rain_scale=scale, t=t)
+        with orc.pow_mode("libm"):
+            orc.hllc_step(o_libm, dt, rain_scale=scale, t=t)
         if scale:
             s = float(rain.astype(np.float64).sum()) * dt * 25.0
             led_g.add_inflow(s)
             led_o.add_inflow(s)
         t += dt
     d = _cmp(g, o)
+    d_libm = _cmp(g, o_libm)
     rg, ro = led_g.record(g, t), led_o.record(o, t)
     assert rg.outflow_cum == pytest.approx(ro["outflow_cum"], rel=1e-6, abs=1e-6)
     assert rg.rel_error < 1e-6
-    print(f"\n[hllc edges] max|dU| {d} outflow {rg.outflow_cum:.4f} mass {rg.rel_error:.2e}")
+    print(f"\n[hllc edges] max|dU| {d} (libm oracle {d_libm}) outflow {rg.outflow_cum:.4f} "
+          f"mass {rg.rel_error:.2e}")
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_uniform_rain(rbs, orc, compensated):
+    """hllc.py:1351-1356: uniform rain, added plainly (_add_uniform_rain, a24) or by
+    the Kahan kernel when the source compensation is armed; h, hu, hv and the Kahan
+    debt against the oracle, with the storm stopping half way."""
+    rng = np.random.default_rng(14)
+    ny, nx = 40, 72
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.008 * (nx - xx) + 0.15 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    d0 = np.maximum(0.4 - z, 0.0).astype(np.float32)
+    g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+    o = orc.State(z, 5.0, depth=d0, manning=0.03)
+    for s in (g, o):
+        s.set_open_boundaries({"west": "open"})
+        if compensated:
+            s.arm_source_compensation()
+    rain = 30.0 / 1000 / 3600
+    t = _run_pair(rbs, orc, g, o, 150, dt_max=2.0, rain=rain)
+    _run_pair(rbs, orc, g, o, 150, dt_max=2.0, rain=0.0, t0=t)
+    d = _cmp(g, o)
+    if compensated:
+        np.testing.assert_array_equal(g.h_comp.numpy(), o.h_comp)
+        assert np.any(o.h_comp != 0.0)
+    assert float(g.h.numpy().astype(np.float64).sum()) > float(d0.astype(np.float64).sum())
+    print(f"\n[hllc uniform rain, compensated={compensated}] max|dU| {d}")
 
 
 def test_wet_bed_stoker_on_gpu(rbs, orc):
