@@ -54,17 +54,18 @@ static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo),
  * p/3 - d, by a fixed sequence of IEEE basic operations (fused multiply-adds only where
  * written as fmaf/fma, which are correctly rounded), so it is the same bits on every
  * IEEE machine; the HIP kernels evaluate the same sequence:
- *   x^(-1/3): a bit-pattern seed, two float32 Newton steps (relative error < 2e-5),
- *     one float64 step of third order, b (1 + r/3 + 2r^2/9 + 14r^3/81), r = 1 - x b^3
- *     formed exactly by an fma; x^(p/3) formed from it in float64;
- *   x^(-d) = exp(-d ln x) = 1 - u + u^2/2, u = d ln x, with ln x from a float32
- *     polynomial (3e-8 absolute, so u is good to 2.4e-15 relative);
- *   the product rounded once.
+ *   y ~ x^(-1/3): a bit-pattern seed, two float32 Newton steps (relative error < 2e-5);
+ *     r = 1 - x y^3 exactly in float64;
+ *   x^e = x^q y^2 (1 - r)^(-2/3) x^(-d), q = 3, 2, 1 for p = 7, 4, 1: the binomial
+ *     series of (1 - r)^(-2/3) to r^3 and exp(-u) = 1 - u + u^2/2, u = d ln x, merged
+ *     into one factor 1 + g; ln x from a float32 polynomial (3e-8 absolute, so u is good
+ *     to 2.4e-15 relative);
+ *   x^q y^2 (1 + g) in float64, rounded once.
  * Mode 0 calls the C library's powf with the float32 exponents, as the reference's
  * Warp-CPU runs did; its published round-off-level mass residuals reproduce with it.
  * tests/test_oracle_pow.py checks mode 1 against the exact powers and against powl.
  */
-static inline double inv_cbrt64(float x) {
+static inline float cbrt_seed(float x) {
   uint32_t i;
   memcpy(&i, &x, 4);
   uint32_t j = 0x54a2fa8cu - i / 3u;
@@ -75,23 +76,18 @@ static inline double inv_cbrt64(float x) {
     float r = fmaf(-(x * y), y * y, 1.0f);
     y = fmaf(y * r, third, y);
   }
-  double a = (double)x, b = (double)y;
-  double r = fma(-(a * b), b * b, 1.0); /* a*b and b*b are exact */
-  double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
-  return fma(b * r, c, b);
+  return y;
 }
-/* ln x, x = 2^k m normal with m in [sqrt(1/2), sqrt(2)), k shifted by kadj:
- * ln(1 + t) = t - t^2/2 + t^3 R(t), t = m - 1 (exact), R of degree 6 fitted to the
- * remainder on that range; float32, then k ln 2 in float64.  3e-8 absolute. */
+/* ln x, x = 2^k m normal with m in [sqrt(1/2), sqrt(2)), k shifted by kadj: the bit
+ * pattern offset by that of sqrt(1/2) splits off k and m; ln(1 + t) = t - t^2/2 +
+ * t^3 R(t), t = m - 1 (exact), R of degree 6 fitted to the remainder on that range;
+ * float32, then k ln 2 in float64.  3e-8 absolute. */
 static inline double ln_coarse(float x, int kadj) {
   uint32_t i;
   memcpy(&i, &x, 4);
-  int k = (int)(i >> 23) - 127 + kadj;
-  uint32_t mb = (i & 0x007fffffu) | 0x3f800000u;
-  if (mb > 0x3fb504f3u) {
-    mb -= 0x00800000u;
-    k += 1;
-  }
+  int32_t ix = (int32_t)(i - 0x3f3504f3u);
+  int k = (ix >> 23) + kadj;
+  uint32_t mb = ((uint32_t)ix & 0x007fffffu) + 0x3f3504f3u;
   float m;
   memcpy(&m, &mb, 4);
   float t = m - 1.0f;
@@ -116,17 +112,22 @@ static float pow_third(float x, int p) {
     xs = x * 0x1p48f;
     kadj = -48;
   }
-  double a = (double)xs, b = inv_cbrt64(xs), pw;
-  if (p == 7)
-    pw = ((a * a) * a) * (b * b);
-  else if (p == 4)
-    pw = (a * a) * (b * b);
-  else
-    pw = a * (b * b);
-  if (tiny) pw *= p == 7 ? 0x1p-112 : p == 4 ? 0x1p-64 : 0x1p-16;
+  double a = (double)xs, b = (double)cbrt_seed(xs);
+  double b2 = b * b;                 /* exact */
+  double r = fma(-(a * b), b2, 1.0); /* a*b exact */
+  double s2 = r * fma(r * (40.0 / 81.0) + 5.0 / 9.0, r, 2.0 / 3.0);
   double u = d * ln_coarse(xs, kadj);
-  double c = fma(u, fma(0.5, u, -1.0), 1.0);
-  return (float)(pw * c);
+  double g = fma(-u, fma(-0.5, u, s2) + 1.0, s2);
+  double A;
+  if (p == 7)
+    A = ((a * a) * a) * b2;
+  else if (p == 4)
+    A = (a * a) * b2;
+  else
+    A = a * b2;
+  double pw = fma(A, g, A);
+  if (tiny) pw *= p == 7 ? 0x1p-112 : p == 4 ? 0x1p-64 : 0x1p-16;
+  return (float)pw;
 }
 static int g_pow_exact = 1;
 void orc_set_pow_exact(int on) { g_pow_exact = on != 0; }

@@ -35,16 +35,16 @@ __device__ __forceinline__ gptr<T> glob(T *p) {
 // by one fixed sequence of IEEE basic operations (the build sets -ffp-contract=off, so
 // the only fused multiply-adds are the fmaf/fma written here, correctly rounded
 // everywhere), bit-identical on every IEEE machine:
-//  1. x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative error
-//     below 2e-5), then one float64 step of third order -- b (1 + r/3 + 2r^2/9 +
-//     14r^3/81) with r = 1 - x b^3 formed exactly by an fma -- within a few float64
-//     ulps; x^(p/3) formed from it in float64;
-//  2. the factor x^(-d) = exp(-d ln x) = 1 - u + u^2/2, u = d ln x (|u| < 1e-5 over
-//     the float32 range, so the next term is below 2e-16), with ln x to 3e-8 absolute
-//     (below): u is then good to 2.4e-15 relative;
-//  3. the product rounded once to float32.
+//  1. y ~ x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative
+//     error below 2e-5); then r = 1 - x y^3 exactly in float64 (x y and y^2 are exact);
+//  2. x^(p/3 - d) = x^q y^2 (1 - r)^(-2/3) x^(-d), since x^(-2/3) = y^2 (1 - r)^(-2/3),
+//     with q = 3, 2, 1 for p = 7, 4, 1: the binomial series of (1 - r)^(-2/3) to r^3 (the next term
+//     is below 6e-18) and x^(-d) = exp(-u) = 1 - u + u^2/2, u = d ln x (|u| < 1e-5 over
+//     the float32 range), merged into one factor 1 + g, with ln x to 3e-8 absolute
+//     (below), so u is good to 2.4e-15 relative;
+//  3. x^q y^2 (1 + g) in float64, rounded once to float32.
 // Float64 here buys the last float32 bit of a power; every field stays float32.
-__device__ __forceinline__ double inv_cbrt64(float x) {
+__device__ __forceinline__ float cbrt_seed(float x) {  // x^(-1/3) to 2e-5
   const uint32_t j = 0x54a2fa8cu - __float_as_uint(x) / 3u;
   float y = __uint_as_float(j);
   const float third = (float)(1.0 / 3.0);
@@ -53,26 +53,16 @@ __device__ __forceinline__ double inv_cbrt64(float x) {
     const float r = fmaf(-(x * y), y * y, 1.0f);
     y = fmaf(y * r, third, y);
   }
-  const double a = (double)x, b = (double)y;
-  const double r = fma(-(a * b), b * b, 1.0);  // a*b and b*b are exact
-  // a product and a sum (not an fma) for the r^2 coefficient: no constant pair for
-  // the loop to keep in vector registers
-  const double c = fma(r * (14.0 / 81.0) + 2.0 / 9.0, r, 1.0 / 3.0);
-  return fma(b * r, c, b);
+  return y;
 }
-// ln x for a positive normal float32 x = 2^k m, m in [sqrt(1/2), sqrt(2)) (k adjusted
-// by kadj), to 3e-8 absolute: ln(1 + t) = t - t^2/2 + t^3 R(t), t = m - 1 exact, with
-// R a degree-6 polynomial fitted to the remainder on that range (error 7e-9), in
-// float32; k ln 2 added in float64.
+// ln x for a positive normal float32 x = 2^k m, m in [sqrt(1/2), sqrt(2)) (k shifted by
+// kadj), to 3e-8 absolute: the bit pattern offset by that of sqrt(1/2) gives k and m;
+// ln(1 + t) = t - t^2/2 + t^3 R(t), t = m - 1 exact, with R a degree-6 polynomial fitted
+// to the remainder on that range (error 7e-9), in float32; k ln 2 added in float64.
 __device__ __forceinline__ double ln_coarse(float x, int kadj) {
-  const uint32_t i = __float_as_uint(x);
-  int k = (int)(i >> 23) - 127 + kadj;
-  uint32_t mb = (i & 0x007fffffu) | 0x3f800000u;
-  if (mb > 0x3fb504f3u) {  // m > sqrt(2): use m / 2
-    mb -= 0x00800000u;
-    k += 1;
-  }
-  const float t = __uint_as_float(mb) - 1.0f;
+  const int32_t ix = (int32_t)(__float_as_uint(x) - 0x3f3504f3u);
+  const int k = (ix >> 23) + kadj;
+  const float t = __uint_as_float(((uint32_t)ix & 0x007fffffu) + 0x3f3504f3u) - 1.0f;
   float r = 0.08672088310927706f;
   r = fmaf(r, t, -0.14379001534630725f);
   r = fmaf(r, t, 0.1497802208269855f);
@@ -101,15 +91,21 @@ __device__ __forceinline__ float pow_third(float x) {
       kadj = -48;
     }
   }
-  const double a = (double)xs, b = inv_cbrt64(xs);
-  double p;
-  if (P == 7) p = ((a * a) * a) * (b * b);
-  else if (P == 4) p = (a * a) * (b * b);
-  else p = a * (b * b);
-  if (SUB && tiny) p *= (P == 7 ? 0x1p-112 : P == 4 ? 0x1p-64 : 0x1p-16);
+  const double a = (double)xs, b = (double)cbrt_seed(xs);
+  const double b2 = b * b;
+  const double r = fma(-(a * b), b2, 1.0);
+  // a product and a sum (not an fma) for the r^2 coefficient: one constant per
+  // instruction, none kept in vector registers
+  const double s2 = r * fma(r * (40.0 / 81.0) + 5.0 / 9.0, r, 2.0 / 3.0);
   const double u = d * ln_coarse(xs, kadj);
-  const double c = fma(u, fma(0.5, u, -1.0), 1.0);
-  return (float)(p * c);
+  const double g = fma(-u, fma(-0.5, u, s2) + 1.0, s2);
+  double A;
+  if (P == 7) A = ((a * a) * a) * b2;
+  else if (P == 4) A = (a * a) * b2;
+  else A = a * b2;
+  double p = fma(A, g, A);
+  if (SUB && tiny) p *= (P == 7 ? 0x1p-112 : P == 4 ? 0x1p-64 : 0x1p-16);
+  return (float)p;
 }
 // wp.pow(h, 7.0/3.0) for a depth h >= H_DRY (friction.py:31)
 __device__ __forceinline__ float pow_e73(float x) { return pow_third<7, false>(x); }
