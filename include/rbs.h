@@ -255,6 +255,15 @@ typedef struct rbs_release {
  * ledger's release_out).  Fills *out; the state's CFL reduction is marked stale. */
 int rbs_reservoir_release(rbs_state *st, const rbs_reservoir *r, double t, double dt_slow,
                           rbs_release *out);
+/* rbs_reservoir_release with no device allocation, for callers that cannot allocate
+ * (code under graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same kernels and results; the record, the hand-off
+ * between launches and the block partials live in a 32 KiB device array the library
+ * holds on each GPU until the process ends (a release needs 56 bytes plus 8 per pool row,
+ * at most 2048 rows' worth, and RBS_E_RANGE is returned beyond 32 KiB).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several host threads) race. */
+int rbs_reservoir_release_noalloc(rbs_state *st, const rbs_reservoir *r, double t,
+                                  double dt_slow, rbs_release *out);
 
 /* ---- ledger and device-driven runs -------------------------------------------------- */
 /* MassLedger inputs: device volume, sink totals, clock (massbalance.py:109-135). */

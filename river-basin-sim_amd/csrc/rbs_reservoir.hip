@@ -14,6 +14,7 @@
 // round-robin, per-block partials), the scale and the delivery in one block each, all
 // sums float64 in a fixed order; only the 32-byte record comes back.
 #include <cmath>
+#include <mutex>
 
 #include "rbs_device.hpp"
 
@@ -171,13 +172,26 @@ __global__ __launch_bounds__(kDeliverThreads) void resv_deliver_kernel(ResvArgs 
   }
 }
 
+// The no-allocation variant's scratch: the record, the hand-off between launches and
+// the block partials of one release, in a file-scope device array (one per GPU, held
+// until the process ends).  Host code uses only the per-device address that
+// hipGetSymbolAddress returns for it, never the array's name.
+constexpr size_t kResvScratchMax = 32768;
+static __device__ __align__(256) unsigned char scratch[kResvScratchMax];
+
+// Bytes of scratch one release uses: the record, the hand-off, one partial per block.
+static size_t release_scratch_bytes(int nb) {
+  return sizeof(rbs_release) + sizeof(ResvScratch) + sizeof(double) * (size_t)nb;
+}
+
 }  // namespace rbs
 
 using namespace rbs;
 
-extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, double t,
-                                     double dt_slow, rbs_release *out) {
-  RBS_API_SHARED;
+namespace {
+
+int check_release(const rbs_state *s, const rbs_reservoir *r, double dt_slow,
+                  const rbs_release *out) {
   if (!s || !r || !out) return set_error("null argument"), RBS_E_ARG;
   auto box_ok = [&](const int32_t *b) {
     return 0 <= b[0] && b[0] <= b[2] && b[2] < s->ny && 0 <= b[1] && b[1] <= b[3] &&
@@ -194,12 +208,21 @@ extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, doubl
   if (r->rule != RBS_RULE_FIXED && r->rule != RBS_RULE_TARGET_STAGE)
     return set_error("unknown release rule %d", r->rule), RBS_E_ARG;
   if (!(dt_slow >= 0.0) || !std::isfinite(dt_slow)) return set_error("bad dt_slow %g", dt_slow), RBS_E_ARG;
-  if (!s->resv_rec) {
-    // the record, the scratch and the block partials, kept for the state's life
-    const size_t bytes = sizeof(rbs_release) + sizeof(ResvScratch) + sizeof(double) * kResvBlocks;
-    RBS_HIP(hipMalloc(&s->resv_rec, bytes));
-  }
-  rbs_release *d_rec = s->resv_rec;
+  return RBS_OK;
+}
+
+// Blocks of the pool passes: the pool's rows dealt round-robin, at most kResvBlocks.
+int release_blocks(const rbs_reservoir *r) {
+  const int pool_rows = r->pool[2] - r->pool[0] + 1;
+  return pool_rows < kResvBlocks ? pool_rows : kResvBlocks;
+}
+
+// The four launches of one release on the state's stream, with `base` the device
+// scratch of release_scratch_bytes(nb) bytes (record, hand-off, partials), then the
+// record back to the host.
+int launch_release(rbs_state *s, const rbs_reservoir *r, double t, double dt_slow,
+                   rbs_release *out, void *base) {
+  rbs_release *d_rec = static_cast<rbs_release *>(base);
   ResvScratch *sc = reinterpret_cast<ResvScratch *>(d_rec + 1);
   double *part = reinterpret_cast<double *>(sc + 1);
   ResvArgs a{};
@@ -211,8 +234,7 @@ extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, doubl
   a.dt_slow = dt_slow;
   a.area = s->dx * s->dx;
   a.pitch = s->pitch;
-  const int pool_rows = r->pool[2] - r->pool[0] + 1;
-  const int nb = pool_rows < kResvBlocks ? pool_rows : kResvBlocks;
+  const int nb = release_blocks(r);
   hipLaunchKernelGGL(resv_sum_kernel, dim3(nb), dim3(kResvThreads), 0, s->stream, a, part);
   hipLaunchKernelGGL(resv_scale_kernel, dim3(1), dim3(kResvThreads), 0, s->stream, a, part, nb, sc);
   hipLaunchKernelGGL(resv_withdraw_kernel, dim3(nb), dim3(kResvThreads), 0, s->stream, a, sc, part);
@@ -225,4 +247,60 @@ extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, doubl
   if (e != hipSuccess) return hip_fail(e, "reservoir release");
   s->speed_scheme = -1;  // depths moved: the cached CFL reduction and volume are stale
   return RBS_OK;
+}
+
+// The address of `scratch` on each device, looked up once with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_at[kMaxDevices] = {};
+
+int scratch_on(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error("device %d is beyond the %d the library tracks", device, kMaxDevices),
+           RBS_E_RANGE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_at[device]) {
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess && prev != device) e = hipSetDevice(device);
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev >= 0 && prev != device) {
+      const hipError_t e2 = hipSetDevice(prev);
+      if (e == hipSuccess) e = e2;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error("no device address for the release scratch"), RBS_E_STATE;
+    g_scratch_at[device] = p;
+  }
+  *out = g_scratch_at[device];
+  return RBS_OK;
+}
+
+}  // namespace
+
+extern "C" int rbs_reservoir_release(rbs_state *s, const rbs_reservoir *r, double t,
+                                     double dt_slow, rbs_release *out) {
+  RBS_API_SHARED;
+  if (int rc = check_release(s, r, dt_slow, out)) return rc;
+  if (!s->resv_rec) {
+    // the record, the scratch and the block partials, kept for the state's life
+    RBS_HIP(hipMalloc(&s->resv_rec, release_scratch_bytes(kResvBlocks)));
+  }
+  return launch_release(s, r, t, dt_slow, out, s->resv_rec);
+}
+
+extern "C" int rbs_reservoir_release_noalloc(rbs_state *s, const rbs_reservoir *r, double t,
+                                             double dt_slow, rbs_release *out) {
+  RBS_API_SHARED;
+  if (int rc = check_release(s, r, dt_slow, out)) return rc;
+  const size_t need = release_scratch_bytes(release_blocks(r));
+  if (need > kResvScratchMax)
+    return set_error("a release needs %zu bytes of scratch, more than the %zu the no-allocation "
+                     "entry point holds",
+                     need, kResvScratchMax),
+           RBS_E_RANGE;
+  void *base = nullptr;
+  if (int rc = scratch_on(s->device, &base)) return rc;
+  return launch_release(s, r, t, dt_slow, out, base);
 }

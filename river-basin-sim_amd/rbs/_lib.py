@@ -138,6 +138,8 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_frames_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "rbs_reservoir_release": (ctypes.c_int, [_vp, ctypes.POINTER(Reservoir), _dbl, _dbl,
                                               ctypes.POINTER(Release)]),
+    "rbs_reservoir_release_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(Reservoir), _dbl, _dbl,
+                                              ctypes.POINTER(Release)]),
     "rbs_set_chan_variant": (ctypes.c_int, [_vp, _i32]),
     "rbs_chan_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_chan_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32]),
