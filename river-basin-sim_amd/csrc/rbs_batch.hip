@@ -352,15 +352,23 @@ int rbs_batch_launch(rbs_batch *b, int32_t steps) {
   if (b->in_flight) return set_error("an advance is in flight: collect it first"), RBS_E_STATE;
   const auto t0 = std::chrono::steady_clock::now();
   bool rebuilt = false;
+  // Every runtime call below runs under the library lock, shared, as in every other
+  // entry point, so that no capture on another thread overlaps it; the capture takes
+  // the lock exclusively.
+  std::shared_lock<std::shared_mutex> api(api_mutex());
   if (b->dirty) {
-    RBS_API_SHARED;
     int rc = rebuild(b);
     if (rc) return rc;
     rebuilt = true;
   }
   if (!b->order.empty() && !b->graph) {
-    RBS_API_CAPTURE;  // no other thread's runtime calls during the capture
-    int rc = capture(b);
+    api.unlock();
+    int rc;
+    {
+      RBS_API_CAPTURE;  // no other thread's runtime calls during the capture
+      rc = capture(b);
+    }
+    api.lock();
     if (rc) return rc;
     rebuilt = true;
   }
@@ -384,6 +392,7 @@ int rbs_batch_launch(rbs_batch *b, int32_t steps) {
 }
 
 int rbs_batch_poll(rbs_batch *b, int32_t *done) {
+  RBS_API_SHARED;
   if (!b || !done) return set_error("null argument"), RBS_E_ARG;
   if (!b->in_flight) {
     *done = 1;
@@ -402,6 +411,7 @@ int rbs_batch_poll(rbs_batch *b, int32_t *done) {
 int rbs_batch_collect(rbs_batch *b, int32_t *n_active) {
   if (!b) return set_error("null batch"), RBS_E_ARG;
   if (b->in_flight) {
+    RBS_API_SHARED;  // as rbs_run_until waits on its stream
     RBS_HIP(hipEventSynchronize(b->done_ev));
     b->in_flight = false;
     float ms = 0.0f;

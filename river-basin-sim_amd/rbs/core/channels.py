@@ -105,61 +105,62 @@ class ChannelState:
         return self._f(_lib.F_QY_FP, self.state.grid.qy_shape)
 
     def summary(self) -> str:
-        return (
-            f"{self.channel_cells} channel cells, width <= {self.width_max:.1f} m, "
-            f"bank-full depth <= {self.depth_max:.2f} m"
-        )
+        return (f"sub-grid channels in {self.channel_cells} cells (widths up to "
+                f"{self.width_max:.1f} m, bank-full depths up to {self.depth_max:.2f} m)")
 
     def as_attrs(self) -> dict:
-        return {
-            "channel_cells": self.channel_cells,
-            "width_max_m": self.width_max,
-            "depth_max_m": self.depth_max,
-        }
+        # the result store's attribute keys for the channel geometry (channels.py:205-217)
+        keys = ("channel_cells", "width_max_m", "depth_max_m")
+        return dict(zip(keys, (self.channel_cells, self.width_max, self.depth_max)))
+
+
+def _channel_field(name: str, values, shape) -> np.ndarray:
+    if np.shape(values) != shape:
+        raise ChannelError(f"the channel {name} field has shape {np.shape(values)}; the grid "
+                           f"is {shape}")
+    return np.array(values, dtype=np.float32, order="C", copy=True)
 
 
 def validate_geometry(width, depth, manning, grid: Grid):
-    """Check channel fields against the grid and normalise "no channel" to
-    ``w = d = 0`` (channels.py:214-286); returns float32 copies ``(w, d, n)``."""
-    ny, nx = grid.shape
-    for name, arr in (("width", width), ("depth", depth), ("manning", manning)):
-        if np.shape(arr) != (ny, nx):
-            raise ChannelError(f"channel {name} shape {np.shape(arr)} != grid {(ny, nx)}")
-    w = np.array(width, dtype=np.float32, order="C", copy=True)
-    d = np.array(depth, dtype=np.float32, order="C", copy=True)
-    n = np.array(manning, dtype=np.float32, order="C", copy=True)
+    """The channel geometry ``arm_channels`` accepts (channels.py:214-286), as float32
+    copies ``(w, d, n)``; a cell without a channel comes back as w = d = 0.
 
-    if not (np.isfinite(w).all() and np.isfinite(d).all()):
-        raise ChannelError("channel width/depth fields contain non-finite values")
-    if (w < 0).any() or (d < 0).any():
-        raise ChannelError("channel width and depth must be >= 0 everywhere")
-    wide = w > grid.dx  # the float32 width against the float64 cell size
-    if wide.any():
+    Refused: fields of another shape than the grid, non-finite or negative widths and
+    depths, a channel wider than its cell (a sub-grid channel lives inside one), a
+    width without a depth or the reverse, and a channel cell without a positive
+    roughness.  A channel far narrower than its cell is legal but sets the column depth
+    dx/w and with it the timestep, so it is warned about."""
+    shape = grid.shape
+    w = _channel_field("width", width, shape)
+    d = _channel_field("depth", depth, shape)
+    n = _channel_field("manning", manning, shape)
+    if not np.isfinite(w).all() or not np.isfinite(d).all():
+        raise ChannelError("channel width or depth holds non-finite values")
+    if min(float(w.min()), float(d.min())) < 0.0:
+        raise ChannelError("channel width and depth must be >= 0 in every cell")
+    too_wide = w > grid.dx  # float32 widths against the float64 cell size
+    if too_wide.any():
         raise ChannelError(
-            f"{int(wide.sum())} cells have a channel wider than the cell "
-            f"(max {float(w[wide].max()):.1f} m > dx {grid.dx:.1f} m); a sub-grid channel "
-            "must fit inside its cell -- resolve the river or coarsen less"
-        )
-    has = (w > 0) & (d > 0)
-    one_sided = (w > 0) != (d > 0)
-    if one_sided.any():
+            f"a channel wider than the cell in {int(too_wide.sum())} cells (up to "
+            f"{float(w.max()):.1f} m against dx = {grid.dx:.1f} m): a sub-grid channel must "
+            "fit inside its cell, so resolve this river on a finer grid")
+    half = (w > 0) ^ (d > 0)
+    if half.any():
         raise ChannelError(
-            f"{int(one_sided.sum())} cells give a channel width without a bank-full depth "
-            "(or the reverse); a channel needs both, and 'no channel' is width = depth = 0"
-        )
-    if has.any() and (n[has] <= 0).any():
-        raise ChannelError("channel cells need manning > 0")
-    w[~has] = 0.0
-    d[~has] = 0.0
-    if has.any():
-        narrowest = float(w[has].min())
-        if narrowest < grid.dx / WIDTH_WARN_RATIO:
+            f"{int(half.sum())} cells give a channel width without a bank-full depth, or a "
+            "depth without a width; a channel has both, and no channel is width = depth = 0")
+    present = (w > 0) & (d > 0)
+    if present.any() and not (n[present] > 0).all():
+        raise ChannelError("every channel cell needs its own manning > 0")
+    w = np.where(present, w, np.float32(0.0))
+    d = np.where(present, d, np.float32(0.0))
+    if present.any():
+        w_min = float(w[present].min())
+        if w_min < grid.dx / WIDTH_WARN_RATIO:
             warnings.warn(
-                f"narrowest sub-grid channel is {narrowest:g} m in a {grid.dx:g} m cell "
-                f"(1:{grid.dx / max(narrowest, 1e-12):.0f}); the water column -- and so the "
-                "timestep -- is set by dx/w, so this will run very slowly",
-                stacklevel=2,
-            )
+                f"a {w_min:g} m channel in {grid.dx:g} m cells: the column depth grows as "
+                "dx/w, and the timestep shrinks with it, so this run will be slow",
+                stacklevel=2)
     return w, d, n
 
 

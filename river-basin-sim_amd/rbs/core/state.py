@@ -240,25 +240,23 @@ class State:
         manning: np.ndarray | float = 0.035,
         device: str = "cuda:0",
     ) -> State:
-        bed = np.ascontiguousarray(bed, dtype=np.float32)
-        ny, nx = bed.shape
-        grid = Grid(ny=ny, nx=nx, dx=float(dx))
-        if np.isscalar(depth):
-            h0 = np.full((ny, nx), float(depth), dtype=np.float32)
-        else:
-            h0 = np.ascontiguousarray(depth, dtype=np.float32)
-            if h0.shape != (ny, nx):
-                raise ValueError(f"depth shape {h0.shape} != bed shape {(ny, nx)}")
-        if np.isscalar(manning):
-            n0 = np.full((ny, nx), float(manning), dtype=np.float32)
-        else:
-            n0 = np.ascontiguousarray(manning, dtype=np.float32)
-            if n0.shape != (ny, nx):
-                raise ValueError(f"manning shape {n0.shape} != bed shape {(ny, nx)}")
-        st = cls(grid, device=device)
-        st.z.assign(bed)
-        st.n.assign(n0)
-        st.h.assign(h0)
+        """A state over ``bed`` with uniform or per-cell initial depth and roughness
+        (state.py:179-226); every face discharge starts at zero."""
+        z = np.ascontiguousarray(bed, dtype=np.float32)
+        shape = z.shape
+
+        def cells(name, value):
+            if np.isscalar(value):
+                return np.full(shape, float(value), dtype=np.float32)
+            arr = np.ascontiguousarray(value, dtype=np.float32)
+            if arr.shape != shape:
+                raise ValueError(f"{name} has shape {arr.shape}; the bed is {shape}")
+            return arr
+
+        h0, n0 = cells("depth", depth), cells("manning", manning)
+        st = cls(Grid(ny=shape[0], nx=shape[1], dx=float(dx)), device=device)
+        for dst, values in ((st.z, z), (st.n, n0), (st.h, h0)):
+            dst.assign(values)
         return st
 
     def depth_numpy(self) -> np.ndarray:

@@ -4,42 +4,51 @@ At each activation the bed moves by the transport integral the fast loop
 accumulated: ``rbs_exner_activate`` runs exner_update (float64, bounded, banked),
 rebuilds ``z = f32(f64(z0) + dz_cum)`` and clears the integral while keeping its
 Kahan debt -- all on the device.  The host keeps the reference's bookkeeping
-(interval check, records, bounds).
+(interval check, records, bounds) and, unless switched off, its per-activation
+morphological Courant diagnostic, which reads a few fields back.
 """
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from rbs import _lib
-from rbs.core.sediment import SedimentError
+from rbs.core.sediment import CourantSummary, SedimentError, courant_summary
 from rbs.scheduler import EPS_T, SlowProcess
 
 
 def bed_change_bounds(shape, *, alluvium_thickness=None, frozen_cells=()):
-    """morphology.py:75-110: per-cell bounds on the cumulative bed change."""
+    """Per-cell bounds (lo, hi) on the cumulative bed change (morphology.py:75-110).
+
+    By default the bed may move freely both ways.  An alluvium thickness t lets a
+    cell erode at most t (t = 0: bedrock, which can still be buried), and a frozen
+    cell may not change at all."""
     ny, nx = shape
-    lo = np.full((ny, nx), -np.inf, dtype=np.float32)
     hi = np.full((ny, nx), np.inf, dtype=np.float32)
-    if alluvium_thickness is not None:
-        thickness = np.asarray(alluvium_thickness, dtype=np.float32)
-        if thickness.ndim and thickness.shape != (ny, nx):
-            raise SedimentError(f"alluvium thickness shape {thickness.shape} != grid {(ny, nx)}")
-        if (thickness < 0).any():
-            raise SedimentError("alluvium thickness must be >= 0 m (0 = bedrock, not a hole)")
-        lo = np.broadcast_to(-thickness, (ny, nx)).astype(np.float32).copy()
+    if alluvium_thickness is None:
+        lo = np.full((ny, nx), -np.inf, dtype=np.float32)
+    else:
+        depth = np.asarray(alluvium_thickness, dtype=np.float32)
+        if depth.ndim and depth.shape != (ny, nx):
+            raise SedimentError(f"the alluvium thickness field has shape {depth.shape}; the "
+                                f"grid is {(ny, nx)}")
+        if (depth < 0).any():
+            raise SedimentError("alluvium thickness must be >= 0 m in every cell (0 is bedrock)")
+        lo = np.ascontiguousarray(np.broadcast_to(-depth, (ny, nx)), dtype=np.float32)
     for i, j in frozen_cells:
         if not (0 <= i < ny and 0 <= j < nx):
-            raise SedimentError(f"frozen cell {(i, j)} is outside the {ny}x{nx} grid")
-        lo[i, j] = 0.0
-        hi[i, j] = 0.0
+            raise SedimentError(f"the frozen cell {(i, j)} lies outside the {ny}x{nx} grid")
+        lo[i, j] = hi[i, j] = 0.0
     return lo, hi
 
 
 @dataclass
 class BedChangeRecord:
+    """One activation (morphology.py:120-160): volumes of solid (m^3), the extremes of
+    the cumulative bed change, and the Courant reductions when they were taken."""
+
     time: float
     interval_s: float
     applied_m3: float
@@ -47,64 +56,78 @@ class BedChangeRecord:
     banked_m3: float
     dz_min_m: float
     dz_max_m: float
+    courant: dict = field(default_factory=dict)
 
     def as_dict(self) -> dict:
-        return dict(self.__dict__)
+        out = {k: v for k, v in self.__dict__.items() if k != "courant"}
+        out.update(self.courant)
+        return out
 
 
 class MorphologyProcess:
-    """Applies Exner on the slow clock (morphology.py:162-307)."""
+    """Exner on the slow clock (morphology.py:162-307)."""
 
     def __init__(self, state, interval_s: float, *, dz_lo=None, dz_hi=None,
-                 name: str = "morphology"):
+                 name: str = "morphology", diagnostics: bool = True):
         sed = state.sediment
         if sed is None:
-            raise SedimentError("morphology needs a state armed with arm_sediment()")
-        if interval_s <= 0:
-            raise SedimentError(f"morphology interval_s must be > 0, got {interval_s}")
+            raise SedimentError("morphology runs on a state armed by arm_sediment(); this one "
+                                "is not")
+        if not interval_s > 0:
+            raise SedimentError(f"the morphology interval_s must be positive, got {interval_s}")
+        grid_shape = state.grid.shape
+        if (dz_lo is None) is not (dz_hi is None):
+            raise SedimentError("bed-change bounds come as a pair: pass both dz_lo and dz_hi")
+        if dz_lo is None:
+            dz_lo, dz_hi = bed_change_bounds(grid_shape)
+        lo = np.ascontiguousarray(dz_lo, dtype=np.float32)
+        hi = np.ascontiguousarray(dz_hi, dtype=np.float32)
+        if lo.shape != grid_shape or hi.shape != grid_shape:
+            raise SedimentError(f"bed-change bounds of shape {lo.shape} and {hi.shape} do not "
+                                f"match the grid {grid_shape}")
+        if (lo > hi).any():
+            raise SedimentError("the bed-change bounds cross: dz_lo exceeds dz_hi somewhere")
         self.state, self.sediment = state, sed
         self.interval_s = float(interval_s)
         self.name = name
+        self.diagnostics = bool(diagnostics)
         self.records: list[BedChangeRecord] = []
-        shape = state.grid.shape
-        if (dz_lo is None) != (dz_hi is None):
-            raise SedimentError("give both dz_lo and dz_hi, or neither")
-        if dz_lo is None:
-            dz_lo, dz_hi = bed_change_bounds(shape)
-        lo = np.ascontiguousarray(dz_lo, dtype=np.float32)
-        hi = np.ascontiguousarray(dz_hi, dtype=np.float32)
-        if lo.shape != shape or hi.shape != shape:
-            raise SedimentError(f"bed-change bounds {lo.shape}/{hi.shape} != grid {shape}")
-        if (lo > hi).any():
-            raise SedimentError("bed-change bounds cross: dz_lo must be <= dz_hi everywhere")
+        self.peak_courant = 0.0
         from rbs.core.state import DeviceField
 
-        DeviceField(state, _lib.F_DZ_LO, shape).assign(lo)
-        DeviceField(state, _lib.F_DZ_HI, shape).assign(hi)
-        frozen = (lo == 0.0) & (hi == 0.0)
-        self.frozen_cells = int(frozen.sum())
-        self.floored_cells = int((np.isfinite(lo) & ~frozen).sum())
+        DeviceField(state, _lib.F_DZ_LO, grid_shape).assign(lo)
+        DeviceField(state, _lib.F_DZ_HI, grid_shape).assign(hi)
+        still = (lo == 0.0) & (hi == 0.0)
+        self.frozen_cells = int(still.sum())
+        self.floored_cells = int((np.isfinite(lo) & ~still).sum())
 
     def as_slow_process(self) -> SlowProcess:
         return SlowProcess(name=self.name, interval=self.interval_s, advance=self.advance)
 
     def advance(self, t: float, dt_slow: float) -> BedChangeRecord:
+        """Apply the integral accumulated over ``dt_slow`` (one interval) to the bed."""
         if dt_slow > self.interval_s * (1.0 + 1e-6) + EPS_T:
             raise SedimentError(
-                f"morphology was handed {dt_slow} s to apply but is configured for "
-                f"{self.interval_s} s: more than one interval has collapsed into a single "
-                "bed update")
+                f"{dt_slow} s handed to a morphology process whose interval is "
+                f"{self.interval_s} s: several intervals would have collapsed into one bed "
+                "update")
         st, sed, g = self.state, self.sediment, self.state.grid
         before = sed.totals()["sum"]
+        dz_before = sed.dz_cum.numpy() if self.diagnostics else None
         _lib.call("rbs_exner_activate", st._handle, float(sed.porosity))
-        solid = g.cell_area * (1.0 - sed.porosity)
-        tot = sed.totals()  # device reductions: the bed-change fields stay on the GPU
-        cumulative = tot["sum"]
+        per_cell = g.cell_area * (1.0 - sed.porosity)
+        tot = sed.totals()  # reduced on the device: the bed-change fields stay there
+        courant: dict = {}
+        if self.diagnostics:
+            cs: CourantSummary = courant_summary(st, sed.dz_cum.numpy() - dz_before,
+                                                 float(dt_slow))
+            self.peak_courant = max(self.peak_courant, cs.courant)
+            courant = cs.as_dict()
         rec = BedChangeRecord(time=float(t), interval_s=float(dt_slow),
-                              applied_m3=(cumulative - before) * solid,
-                              cumulative_m3=cumulative * solid,
-                              banked_m3=tot["unapplied"] * solid,
-                              dz_min_m=tot["min"], dz_max_m=tot["max"])
+                              applied_m3=(tot["sum"] - before) * per_cell,
+                              cumulative_m3=tot["sum"] * per_cell,
+                              banked_m3=tot["unapplied"] * per_cell,
+                              dz_min_m=tot["min"], dz_max_m=tot["max"], courant=courant)
         self.records.append(rec)
         return rec
 
@@ -114,4 +137,5 @@ class MorphologyProcess:
 
     def as_attrs(self) -> dict:
         return {**self.sediment.as_attrs(), "interval_s": self.interval_s,
-                "frozen_cells": self.frozen_cells, "floored_cells": self.floored_cells}
+                "frozen_cells": self.frozen_cells, "floored_cells": self.floored_cells,
+                "peak_courant": self.peak_courant}

@@ -104,7 +104,9 @@ def build_state(req: Request, tune_kernel: bool = True):
     procs = []
     if req.d50 is not None:
         arm_sediment(st, req.d50, req.porosity)
-        morph = MorphologyProcess(st, req.interval_s)
+        # a served request records its bed-change volumes; the Courant read-back (five
+        # fields per activation) is left to runs that ask for it
+        morph = MorphologyProcess(st, req.interval_s, diagnostics=False)
         procs = [morph.as_slow_process()]
     if tune_kernel:
         from rbs.autotune import tune
