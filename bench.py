@@ -909,18 +909,11 @@ def bench_channels(args) -> int:
 
 
 def valley(n: int, dx: float, seed: int):
-    """Synthetic valley: 40 m x 3 m trapezoidal channel (1:2 banks) on slope 5e-4
-    down the rows, 2 m fBm floodplain relief."""
+    """The HLLC valley's bed and channel columns (rbs.terrain.valley)."""
     from rbs import terrain
 
-    yy = np.arange(n)[:, None].astype(np.float64)
-    xx = np.arange(n)[None, :].astype(np.float64)
-    c = n / 2.0
-    dist = np.abs(xx - c) * dx
-    chan = np.clip(3.0 - np.maximum(dist - 20.0, 0.0) * 0.5, 0.0, 3.0)
-    relief = terrain.fbm_dem(n, n, hurst=0.8, relief=2.0, seed=seed)
-    bed = 5e-4 * dx * (n - yy) - chan + 3.0 + relief * (dist > 26.0)
-    return bed.astype(np.float32), [int(j) for j in range(n) if abs(j - c) * dx <= 20.0]
+    v = terrain.valley(n, dx=dx, seed=seed)
+    return v["bed"], v["channel_cols"]
 
 
 def bench_hllc(args) -> int:

@@ -165,7 +165,11 @@ static inline float manning_denominator(float q_prev, float h_flow, float n, flo
 long orc_pow_sweep(int kind, uint32_t lo, uint32_t hi, int libm) {
   const float ex[4] = {(float)(7.0 / 3.0), (float)(4.0 / 3.0), (float)(1.0 / 3.0), 1.5f};
   long bad = 0;
-#pragma omp parallel for reduction(+ : bad) schedule(static)
+  int procs = 1;
+#ifdef _OPENMP
+  procs = omp_get_num_procs(); /* a test sweep: every processor, whatever the passes use */
+#endif
+#pragma omp parallel for reduction(+ : bad) schedule(static) num_threads(procs)
   for (long long u = (long long)lo; u < (long long)hi; ++u) {
     uint32_t bits = (uint32_t)u;
     float x;

@@ -191,3 +191,19 @@ def reach_basin(n: int, *, dx: float = 100.0, seed: int = 20261203) -> dict:
     return {"bed": bed, "manning": manning, "rain": rain.astype(np.float32),
             "infil": infil.astype(np.float32), "dx": dx, "width": width, "depth": depth,
             "chan_manning": np.full((n, n), 0.03, np.float32)}
+
+
+def valley(n: int, *, dx: float = 5.0, seed: int = 20261204) -> dict:
+    """The HLLC valley: an n x n grid of dx cells, a trapezoidal main channel 40 m wide
+    and 3 m deep with 1:2 banks down the middle column, the bed falling 5e-4 down the
+    rows (north to south), and 2 m of fBm relief on the floodplain clear of the channel.
+    Returns the bed and the channel's columns (where the inflow enters on row 0)."""
+    rows = np.arange(n, dtype=np.float64)[:, None]
+    cols = np.arange(n, dtype=np.float64)[None, :]
+    centre = n / 2.0
+    off = np.abs(cols - centre) * dx  # metres from the channel axis
+    channel = np.clip(3.0 - np.maximum(off - 20.0, 0.0) * 0.5, 0.0, 3.0)
+    relief = fbm_dem(n, n, hurst=0.8, relief=2.0, seed=seed)
+    bed = 5e-4 * dx * (n - rows) - channel + 3.0 + relief * (off > 26.0)
+    inflow_cols = [int(j) for j in range(n) if abs(j - centre) * dx <= 20.0]
+    return {"bed": bed.astype(np.float32), "channel_cols": inflow_cols, "dx": dx}

@@ -46,8 +46,8 @@ def test_stage_transport_integral_matches_the_oracle(rbs, orc):
         t += dt
     assert np.abs(g.h.numpy() - o.h).max() <= 1e-5
     sed = g.sediment
-    np.testing.assert_allclose(sed.qs_int_x.numpy(), o.qs_int_x, rtol=2e-5, atol=1e-10)
-    np.testing.assert_allclose(sed.qs_int_y.numpy(), o.qs_int_y, rtol=2e-5, atol=1e-10)
+    np.testing.assert_array_equal(sed.qs_int_x.numpy(), o.qs_int_x)  # one power sequence: bitwise
+    np.testing.assert_array_equal(sed.qs_int_y.numpy(), o.qs_int_y)
     assert np.abs(o.qs_int_x).max() > 0.0
 
 

@@ -39,8 +39,8 @@ def test_transport_integral_and_exner_match_the_oracle(rbs, orc):
         rbs.li.step(g, dt)
         orc.li_step(o, dt)
     sed = g.sediment
-    np.testing.assert_allclose(sed.qs_int_x.numpy(), o.qs_int_x, rtol=2e-5, atol=1e-10)
-    np.testing.assert_allclose(sed.qs_int_y.numpy(), o.qs_int_y, rtol=2e-5, atol=1e-10)
+    np.testing.assert_array_equal(sed.qs_int_x.numpy(), o.qs_int_x)  # one power sequence: bitwise
+    np.testing.assert_array_equal(sed.qs_int_y.numpy(), o.qs_int_y)
     assert np.abs(g.h.numpy() - o.h).max() <= 1e-5
     # one activation on the device vs the oracle's exner on the device's integrals
     qx, qy = sed.qs_int_x.numpy(), sed.qs_int_y.numpy()
