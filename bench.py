@@ -58,6 +58,24 @@ def build_state(n: int, seed: int):
     return st, d
 
 
+def host_threads() -> int:
+    """The host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (a shared host's share of its cores)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(st, d, dt: float, threads: int, budget_s: float) -> dict:
     """Time the oracle (C restatement, -O3, OpenMP) on a copy of the current state."""
     from oracle import oracle as orc
@@ -94,6 +112,8 @@ def cpu_baseline(st, d, dt: float, threads: int, budget_s: float) -> dict:
         "sample": f"{steps} steps of the {o.ny}x{o.nx} spun-up state, {el:.1f} s; "
                   f"1 thread: {out[1][0]:.5f} Gcell-updates/s over {out[1][1]} steps",
         "value_1thread": round(out[1][0], 6),
+        "cpu_model": cpu_model(),
+        "cpus_visible": os.cpu_count(),
     }
 
 
@@ -105,7 +125,13 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=4096, help="grid side (cells)")
     ap.add_argument("--spinup", type=float, default=3600.0, help="simulated s before timing")
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--variant", type=int, default=-1, help="LI kernel variant (-1: autotuned)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="LI kernel variant (-1: the library's choice for the grid)")
+    ap.add_argument("--tune", action="store_true", help="autotune the LI kernel variant")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: skip the ensemble and serving figures")
+    ap.add_argument("--serve-sample", type=int, default=120,
+                    help="requests in the default line's serving sample")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -157,11 +183,16 @@ def main() -> int:
     if args.variant >= 0:
         _lib.call("rbs_set_li_variant", st._handle, args.variant)
         variant = args.variant
-    else:
+    elif args.tune:
         from rbs.autotune import tune
 
         variant = tune(st)
         log(f"[bench] autotuned LI variant {variant}")
+    else:  # the kernel a state of this shape starts on: the same one every run
+        v = ctypes.c_int32()
+        _lib.call("rbs_get_li_variant", st._handle, ctypes.byref(v))
+        variant = int(v.value)
+        log(f"[bench] LI variant {variant} (the library's choice for this shape)")
     cells = args.n * args.n
     rain_sum = float(d["rain"].astype(np.float64).sum())
     end_time = 10800.0
@@ -214,7 +245,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(st, d, dt_k, min(16, os.cpu_count() or 1), args.cpu_budget)
+        cpu = cpu_baseline(st, d, dt_k, host_threads(), args.cpu_budget)
 
     mass = None
     try:
@@ -224,6 +255,11 @@ def main() -> int:
     except Exception:
         pass
 
+    extras = {}
+    if not args.no_extras:
+        st.close()  # the headline state's HBM back before the workloads below
+        extras["ensemble"] = ensemble_figures(args)
+        extras["serving"] = serve_figures(args, args.serve_sample, min(60, args.capacity_sample))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -261,6 +297,7 @@ def main() -> int:
     }
     if sweep:
         out["variant_sweep"] = sweep
+    out.update(extras)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -313,12 +350,17 @@ def bench_copy(args) -> int:
     return 0
 
 
-def bench_ensemble(args) -> int:
-    """Batched ensemble vs the same scenarios run one after another: as plain
+def ensemble_figures(args) -> dict:
+    """The batched ensemble against the same scenarios run one after another: as plain
     per-scenario launches (the reference's tick loop over compute_dt/step) and as
-    device-driven graph-replayed runs.  Every scenario runs its shape's tuned kernel."""
+    device-driven graph-replayed runs.  Every scenario runs its shape's tuned kernel
+    (tuned once per scenario, outside the timings); every result is compared bitwise."""
+    from rbs import _lib
+    from rbs.autotune import tune
+    from rbs.core import local_inertial as li
+    from rbs.core.state import State
     from rbs.driver import DeviceRun
-    from rbs.ensemble import Batch, ensemble_inputs, make_member_state
+    from rbs.ensemble import Batch, ensemble_inputs
     from rbs.scheduler import MultiRateScheduler
 
     t0 = time.time()
@@ -327,24 +369,33 @@ def bench_ensemble(args) -> int:
     log(f"[ensemble] {len(members)} scenarios, {cells / 1e6:.1f} M cells, generated in "
         f"{time.time() - t0:.0f} s")
     end = 3600.0 * args.sim_hours
+    variants: dict[int, int] = {}
 
-    def mk(m):
-        st = make_member_state(m)
+    def state(k):
+        m = members[k]
+        st = State.from_bed(m.bed, m.dx, depth=0.0, manning=m.manning)
+        st.set_rain_field(m.rain)
+        st.set_open_boundaries({e: "open" for e in ("north", "south", "east", "west")})
+        st.arm_source_compensation()
+        if k not in variants:
+            variants[k] = tune(st)
+        else:
+            _lib.call("rbs_set_li_variant", st._handle, variants[k])
+        return st
+
+    def run(k):
+        m = members[k]
         sched = MultiRateScheduler(end_time=end, output_every=end)
-        run = DeviceRun(st, "local_inertial", sched, alpha=0.7, dt_max=30.0, rain_is_field=True,
-                        rain_duration=end, rain_field_sum=float(m.rain.astype(np.float64).sum()))
-        return run
+        return DeviceRun(state(k), "local_inertial", sched, alpha=0.7, dt_max=30.0,
+                         rain_is_field=True, rain_duration=end,
+                         rain_field_sum=float(m.rain.astype(np.float64).sum()))
 
     # plain per-scenario launches: the reference's tick loop (run.py:369-388) over the
     # scheme seam -- compute_dt (one reduction + a device-to-host read), fill_span on
     # the host, one step launch -- one scenario after another
-    from rbs.core import local_inertial as li
-
-    t_plain = 0.0
-    plain_steps = 0
-    keep_plain = {}
-    for k, m in enumerate(members):
-        st = make_member_state(m)
+    t_plain, plain_steps, h_plain = 0.0, 0, []
+    for k in range(len(members)):
+        st = state(k)
         sched = MultiRateScheduler(end_time=end, output_every=end)
         st.sync()
         t0 = time.perf_counter()
@@ -354,27 +405,22 @@ def bench_ensemble(args) -> int:
             plain_steps += 1
         st.sync()
         t_plain += time.perf_counter() - t0
-        if k % 37 == 0:
-            keep_plain[k] = st.h.numpy()
+        h_plain.append(st.h.numpy())
         st.close()
     log(f"[ensemble] plain per-scenario launches: {t_plain:.2f} s, {plain_steps} steps")
-
     # sequential device-driven runs: one scenario at a time, steps replayed from a graph
-    t_seq = 0.0
-    seq_steps = 0
-    keep = {}
-    for k, m in enumerate(members):
-        run = mk(m)
-        run.state.sync()
+    t_seq, seq_steps, same = 0.0, 0, True
+    for k in range(len(members)):
+        r = run(k)
+        r.state.sync()
         t0 = time.perf_counter()
-        seq_steps += run.run_until(end)
-        run.state.sync()
+        seq_steps += r.run_until(end)
+        r.state.sync()
         t_seq += time.perf_counter() - t0
-        if k % 37 == 0:
-            keep[k] = run.state.h.numpy()
-        run.state.close()
+        same = same and np.array_equal(r.state.h.numpy(), h_plain[k])
+        r.state.close()
     log(f"[ensemble] sequential: {t_seq:.2f} s GPU wall, {seq_steps} steps")
-    runs = [mk(m) for m in members]
+    runs = [run(k) for k in range(len(members))]
     for r in runs:
         r.state.sync()
     b = Batch(len(runs))
@@ -384,45 +430,62 @@ def bench_ensemble(args) -> int:
     t_bat = time.perf_counter() - t0
     for sl in slots:
         b.remove(sl)
-    same = all(np.array_equal(runs[k].state.h.numpy(), h) for k, h in keep.items())
-    same = same and all(np.array_equal(keep_plain[k], h) for k, h in keep.items())
-    bat_steps = sum(r.state.ledger().steps for r in runs)
-    log(f"[ensemble] batched: {t_bat:.2f} s wall, {bat_steps} steps, identical={same}")
+    b.close()
+    same = same and all(np.array_equal(r.state.h.numpy(), h) for r, h in zip(runs, h_plain))
+    bat_steps = sum(int(r.state.ledger().steps) for r in runs)
+    updates = sum(m.bed.size * int(r.state.ledger().steps) for m, r in zip(members, runs))
+    for r in runs:
+        r.state.close()
     n = len(members)
-    out = {
-        "metric": METRIC,
-        "value": round(3600.0 * n / t_bat, 1),
-        "unit": "scenarios/h",
-        "n_gpus": 1,
-        "steps": int(bat_steps),
-        "warmup": 0,
-        "ms_per_step": round(1e3 * t_bat / max(1, bat_steps / n), 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"li_ensemble_{n}", "sides": "log-uniform 256..2048",
-                   "dx_m": 10.0, "simulated_s": end, "cells_total": cells},
+    log(f"[ensemble] batched: {t_bat:.2f} s wall, {bat_steps} steps, identical={same}")
+    return {
+        "scenarios": n,
+        "scenarios_per_h": round(3600.0 * n / t_bat, 1),
+        "wall_s": round(t_bat, 3),
+        "sides": "log-uniform 256..2048, dx 10 m", "simulated_s": end, "cells_total": cells,
         "plain_launches_scenarios_per_h": round(3600.0 * n / t_plain, 1),
         "speedup_vs_plain_launches": round(t_plain / t_bat, 3),
         "sequential_device_driven_scenarios_per_h": round(3600.0 * n / t_seq, 1),
         "speedup_vs_sequential_device_driven": round(t_seq / t_bat, 3),
         "bitwise_identical_batched_sequential_plain": bool(same),
-        "batched_cell_updates_per_s": round(
-            sum(m.bed.size * r.state.ledger().steps for m, r in zip(members, runs)) / t_bat / 1e9, 3),
+        "batched_cell_updates_per_s": round(updates / t_bat / 1e9, 3),
+        "steps": bat_steps,
+    }
+
+
+def bench_ensemble(args) -> int:
+    """cfg4 as its own JSON line: scenarios/h of the batch (ensemble_figures)."""
+    e = ensemble_figures(args)
+    out = {
+        "metric": METRIC,
+        "value": e["scenarios_per_h"],
+        "unit": "scenarios/h",
+        "n_gpus": 1,
+        "steps": e["steps"],
+        "warmup": 0,
+        "ms_per_step": round(1e3 * e["wall_s"] / max(1, e["steps"] / e["scenarios"]), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"li_ensemble_{e['scenarios']}", "sides": e["sides"],
+                   "dx_m": 10.0, "simulated_s": e["simulated_s"], "cells_total": e["cells_total"]},
+        **{k: v for k, v in e.items() if k not in ("scenarios_per_h", "steps")},
     }
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_serve(args) -> int:
-    """Continuous-batching server: Poisson arrivals, mixed basins and alluvial reaches."""
+def serve_figures(args, requests: int, capacity_sample: int) -> dict:
+    """The continuous-batching server on ``requests`` Poisson arrivals of the served mix,
+    with p50/p99 time-to-result, and its capacity on the first ``capacity_sample`` of
+    them as one burst against plain per-request launches."""
     from rbs.serve import InputPool, Server, make_requests
 
     t0 = time.time()
     pool = InputPool(seed=20261206)
-    reqs = make_requests(args.requests, rate_per_s=args.rate, seed=20261206, pool=pool,
+    reqs = make_requests(requests, rate_per_s=args.rate, seed=20261206, pool=pool,
                          end_time=3600.0 * args.sim_hours)
     log(f"[serve] {len(reqs)} requests generated in {time.time() - t0:.0f} s")
     variants = None
@@ -433,27 +496,17 @@ def bench_serve(args) -> int:
     srv = Server(capacity=256, chunk=args.chunk, variants=variants)
     stats = srv.serve(reqs, log=log)
     log(f"[serve] {stats}")
-    cap = (serve_capacity(reqs[: args.capacity_sample], args.chunk, variants)
-           if args.capacity_sample > 0 else None)
+    cap = (serve_capacity(reqs[:capacity_sample], args.chunk, variants)
+           if capacity_sample > 0 else None)
     kinds = {}
     for r in srv.results:
         kinds.setdefault(r.kind, []).append(r.time_to_result)
     out = {
-        "metric": METRIC,
-        "value": round(stats["scenarios_per_h"], 1),
-        "unit": "scenarios/h",
-        "n_gpus": 1,
-        "steps": int(sum(r.steps for r in srv.results)),
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"serve_{len(reqs)}_poisson_{args.rate}",
-                   "mix": "70% LI basins 512..4096 dx10, 30% LI+Exner reaches 1024x256..4096x1024 dx2",
-                   "simulated_s": 3600.0 * args.sim_hours},
+        "requests": len(reqs),
+        "arrivals_per_s": args.rate,
+        "mix": "70% LI basins 512..4096 dx10, 30% LI+Exner reaches 1024x256..4096x1024 dx2",
+        "simulated_s": 3600.0 * args.sim_hours,
+        "scenarios_per_h": round(stats["scenarios_per_h"], 1),
         "p50_s": round(stats["p50_s"], 3),
         "p99_s": round(stats["p99_s"], 3),
         "per_kind_p50_s": {k: round(float(np.percentile(v, 50)), 3) for k, v in kinds.items()},
@@ -462,9 +515,33 @@ def bench_serve(args) -> int:
         "peak_live": stats["peak_live"],
         "peak_resident_gb": round(stats["peak_resident_gb"], 2),
         "wall_s": round(stats["wall_s"], 1),
+        "steps": int(sum(r.steps for r in srv.results)),
     }
     if cap:
         out["capacity"] = cap
+    return out
+
+
+def bench_serve(args) -> int:
+    """cfg5 as its own JSON line (serve_figures over ``--requests``)."""
+    f = serve_figures(args, args.requests, args.capacity_sample)
+    out = {
+        "metric": METRIC,
+        "value": f["scenarios_per_h"],
+        "unit": "scenarios/h",
+        "n_gpus": 1,
+        "steps": f["steps"],
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"serve_{f['requests']}_poisson_{args.rate}", "mix": f["mix"],
+                   "simulated_s": f["simulated_s"]},
+        **{k: v for k, v in f.items() if k not in ("scenarios_per_h", "steps", "mix")},
+    }
     print(json.dumps(out), flush=True)
     return 0
 

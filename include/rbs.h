@@ -312,11 +312,13 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active);
 int rbs_time_li_kernel(rbs_state *st, int32_t reps, double dt, double rain, double rain_scale,
                        double *avg_ms);
 /* Select the LI kernel variant, 0 <= variant < rbs_li_variant_count() (a new state
- * starts on the library's default, or on the 16-row row blocks when it is at least
+ * starts on the library's default, or on the 50-row row blocks when it is at least
  * 2048 cells wide).  Returns RBS_E_ARG for an unknown variant.
  * Depths and faces are bitwise identical across variants (the ledger's volume sum
  * may differ in its last bits: its partials follow the tile shape). */
 int rbs_set_li_variant(rbs_state *st, int32_t variant);
+/* The LI kernel variant a state runs (its initial one, or the last set). */
+int rbs_get_li_variant(const rbs_state *st, int32_t *variant);
 int rbs_li_variant_count(int32_t *count);
 /* Shape of a variant: the block's tile of cells (bx x by) and its family
  * (0 = LDS tile, 1 = register strips with a column per lane, 3 = register strips side

@@ -1041,7 +1041,7 @@ static const LiVariant kLiVariants[] = {
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;    // strips of 32 rows
-constexpr int kRowsVariantForWide = 9;  // row blocks of 16 rows, two rows per barrier
+constexpr int kRowsVariantForWide = 11;  // row blocks of 50 rows, two rows per barrier
 
 // ---- family 4: register strips of a state with sub-grid channels ----------------------
 // li_strip's rolling window with the channel forms: each row's cells carry the
@@ -1333,9 +1333,10 @@ static const LiVariant &variant_at(int v) {
 
 int li_variant_count() { return kNumLiVariants; }
 int li_default_variant() { return kDefaultLiVariant; }
-// A new state's variant: the row blocks of 16 rows on grids at least 2048 cells wide
-// (where a ragged last block idles few of their waves; 176-180 us against 215 us for
-// the default at 4096^2), the library default otherwise.
+// A new state's variant: the row blocks of 50 rows on grids at least 2048 cells wide
+// (where a ragged last block idles few of their waves; at 4096^2 they re-read the
+// fewest halo rows, 205 us against 214 us for 16-row blocks and 222 us for the
+// default), the library default otherwise.
 int li_initial_variant(int nx) { return nx >= 2048 ? kRowsVariantForWide : kDefaultLiVariant; }
 int li_wide_batch_variant() { return kRowsVariantForWide; }
 int li_wide_batch_min_nx() { return 2048; }

@@ -678,6 +678,12 @@ int rbs_sync(rbs_state *s) {
   return RBS_OK;
 }
 
+int rbs_get_li_variant(const rbs_state *s, int32_t *variant) {
+  if (!s || !variant) return set_error("null argument"), RBS_E_ARG;
+  *variant = s->li_variant;
+  return RBS_OK;
+}
+
 int rbs_set_li_variant(rbs_state *s, int32_t variant) {
   RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
@@ -967,9 +973,12 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
     const double est = c.dt > 0.0 ? (t_stop - c.t) / c.dt : 1.0;
     int64_t want = (int64_t)std::ceil(std::min(est, 1e9)) + 1;
     want = std::min<int64_t>(want, steps0 + max_steps - c.steps);
-    int64_t launches = (want + chunk - 1) / chunk;
-    launches = std::max<int64_t>(1, std::min<int64_t>(launches, 64));
-    if (!s->graph[0]) {
+    // whole chunks from the graph, and a remainder shorter than a chunk as direct
+    // launches, so that a step limit is met without replaying steps that find the
+    // limit reached and return at once
+    int64_t launches = std::min<int64_t>(want / chunk, 64);
+    const int64_t rest = launches == 64 ? 0 : want % chunk;
+    if (launches > 0 && !s->graph[0]) {
       api.unlock();
       {
         RBS_API_CAPTURE;  // no other thread's runtime calls during the capture
@@ -979,6 +988,14 @@ extern "C" int rbs_run_until(rbs_state *s, const rbs_run_params *p, double t_sto
       if (rc) return rc;
     }
     for (int64_t L = 0; L < launches; ++L) RBS_HIP(hipGraphLaunch(s->graph[0], s->stream));
+    for (int64_t k = 0; k < rest; ++k) {
+      rc = enqueue_driven_step(s, p->scheme, p->limit);
+      if (rc) return rc;
+    }
+    if (launches == 0 && rest == 0) {  // nothing estimated: one step to move the clock
+      rc = enqueue_driven_step(s, p->scheme, p->limit);
+      if (rc) return rc;
+    }
   }
   s->speed_scheme = p->scheme;
   if (steps_taken) *steps_taken = taken;

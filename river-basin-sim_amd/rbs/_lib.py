@@ -125,6 +125,7 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_batch_advance": (ctypes.c_int, [_vp, _i32, _pi32]),
     "rbs_batch_member_active": (ctypes.c_int, [_vp, _i32, _pi32]),
     "rbs_set_li_variant": (ctypes.c_int, [_vp, _i32]),
+    "rbs_get_li_variant": (ctypes.c_int, [_vp, _pi32]),
     "rbs_li_variant_count": (ctypes.c_int, [_pi32]),
     "rbs_li_variant_info": (ctypes.c_int, [_i32, _pi32, _pi32, _pi32, _pi32]),
     "rbs_batch_set_variant": (ctypes.c_int, [_vp, _i32, _i32]),
