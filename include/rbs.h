@@ -152,6 +152,13 @@ int rbs_arm(rbs_state *st, uint32_t arm);
 int rbs_armed(const rbs_state *st, uint32_t *arm);
 /* Device bytes held by the state (footprint for admission control). */
 int rbs_state_bytes(const rbs_state *st, uint64_t *bytes);
+/* Device bytes a state of ny x nx armed with `arm` will hold, without creating it (the
+ * admission check of a server): *fields is what rbs_state_create and rbs_arm allocate
+ * for the fields, which rbs_state_bytes reports for a fresh state; *total adds the
+ * block partials and 1 MiB for the small allocations a run adds later.  No reference
+ * counterpart (run.py:69-91 estimates the same from field sizes). */
+int rbs_state_footprint(int32_t ny, int32_t nx, uint32_t arm, uint64_t *fields,
+                        uint64_t *total);
 
 /* wp.array(...) / .assign(...) and .numpy() on State fields (state.py:200-226).
  * `bytes` must equal the field's dense size.  Uploading h, z or hu/hv marks the

@@ -148,6 +148,56 @@ static int alloc_field(rbs_state *s, void **p, int rows, int elem) {
 #define ALLOC(ptr, rows, elem) \
   do { int rc_ = alloc_field(s, (void **)&(ptr), (rows), (elem)); if (rc_) return rc_; } while (0)
 
+// Bytes of the pitched fields a state of ny rows armed with `arm` holds, row counts
+// and element sizes as rbs_state_create and arm_fields allocate them.
+static uint64_t field_bytes(int ny, int64_t pitch, uint32_t arm) {
+  struct Spec {
+    uint32_t flag;  // 0: every state
+    int count, extra_rows, elem;
+  };
+  static const Spec specs[] = {
+      {0, 4, 0, 4},              // h, qx (ping-pong)
+      {0, 2, 1, 4},              // qy (ping-pong)
+      {0, 2, 0, 4},              // z, n
+      {RBS_ARM_COMP, 1, 0, 4},   // h_comp
+      {RBS_ARM_RAIN, 1, 0, 4},   // rain
+      {RBS_ARM_INFIL, 1, 0, 4},  // infil
+      {RBS_ARM_HLLC, 5, 0, 4},   // hu, hv, h1, hu1, hv1
+      {RBS_ARM_SED, 6, 0, 4},    // d50, qs_int_x, qs_comp_x, z0, dz_lo, dz_hi
+      {RBS_ARM_SED, 2, 1, 4},    // qs_int_y, qs_comp_y
+      {RBS_ARM_SED, 2, 0, 8},    // dz_cum, dz_unapplied
+      {RBS_ARM_CHAN, 7, 0, 4},   // ch_w, ch_d, ch_n, qx_ch and qx_fp (ping-pong)
+      {RBS_ARM_CHAN, 4, 1, 4},   // qy_ch, qy_fp (ping-pong)
+  };
+  uint64_t total = 0;
+  for (const Spec &sp : specs)
+    if (sp.flag == 0 || (arm & sp.flag))
+      total += (uint64_t)sp.count * (uint64_t)(ny + sp.extra_rows) * (uint64_t)pitch * sp.elem;
+  return total;
+}
+
+// Block partials (and dry-tile words) for the largest tiling any variant uses.
+static int max_blocks(rbs_state *s) {
+  const int li0 = s->li_variant, h0 = s->hllc_variant, c0 = s->chan_variant;
+  int nblk = reduce_partials_needed(s);
+  for (int v = 0; v < li_variant_count(); ++v) {
+    s->li_variant = v;
+    nblk = std::max(nblk, li_num_blocks(s));
+  }
+  for (int v = 0; v < hllc_variant_count(); ++v) {
+    s->hllc_variant = v;
+    nblk = std::max(nblk, hllc_num_blocks(s));
+  }
+  for (int v = 0; v < chan_variant_count(); ++v) {
+    s->chan_variant = v;
+    nblk = std::max(nblk, chan_num_blocks(s));
+  }
+  s->li_variant = li0;
+  s->hllc_variant = h0;
+  s->chan_variant = c0;
+  return nblk;
+}
+
 static void invalidate_graph(rbs_state *s) {
   for (int k = 0; k < 2; ++k)
     if (s->graph[k]) {
@@ -292,24 +342,10 @@ int rbs_state_create(int32_t ny, int32_t nx, double dx, uint32_t arm, rbs_state 
     return fail(hip_fail(hipErrorOutOfMemory, "hipHostMalloc(ctl)"));
   if (hipMemsetAsync(s->ctl, 0, sizeof(DevCtl), s->stream) != hipSuccess)
     return fail(hip_fail(hipErrorUnknown, "hipMemset(ctl)"));
-  int nblk = li_num_blocks(s);
-  for (int v = 0; v < li_variant_count(); ++v) {
-    s->li_variant = v;
-    nblk = std::max(nblk, li_num_blocks(s));
-  }
   s->li_variant = li_initial_variant(s->nx);
-  nblk = std::max(nblk, reduce_partials_needed(s));
-  for (int v = 0; v < hllc_variant_count(); ++v) {
-    s->hllc_variant = v;
-    nblk = std::max(nblk, hllc_num_blocks(s));
-  }
   s->hllc_variant = hllc_default_variant();
-  for (int v = 0; v < chan_variant_count(); ++v) {
-    s->chan_variant = v;
-    nblk = std::max(nblk, chan_num_blocks(s));
-  }
   s->chan_variant = chan_default_variant();
-  TRY(ensure_partials(s, nblk));
+  TRY(ensure_partials(s, max_blocks(s)));
   if (arm) TRY(arm_fields(s, arm));
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(hip_fail(hipErrorUnknown, "sync"));
 #undef TRY
@@ -359,6 +395,26 @@ int rbs_arm(rbs_state *s, uint32_t arm) {
 int rbs_armed(const rbs_state *s, uint32_t *arm) {
   if (!s || !arm) return set_error("null argument"), RBS_E_ARG;
   *arm = s->armed;
+  return RBS_OK;
+}
+
+int rbs_state_footprint(int32_t ny, int32_t nx, uint32_t arm, uint64_t *fields,
+                        uint64_t *total) {
+  if (!fields || !total) return set_error("null argument"), RBS_E_ARG;
+  if (ny < 1 || nx < 1) return set_error("bad grid ny=%d nx=%d", ny, nx), RBS_E_ARG;
+  rbs_state probe;  // geometry only: nothing is allocated
+  probe.ny = ny;
+  probe.nx = nx;
+  probe.pitch = ((int64_t)nx + 1 + 63) / 64 * 64;
+  probe.li_variant = li_initial_variant(nx);
+  probe.hllc_variant = hllc_default_variant();
+  probe.chan_variant = chan_default_variant();
+  const uint64_t nblk = (uint64_t)max_blocks(&probe);
+  *fields = field_bytes(ny, probe.pitch, arm);
+  // partials: a float, three doubles and a dry-tile word per block; then the control
+  // block, point-source tables, reduction scratch and staging the state adds later
+  *total = *fields + nblk * (sizeof(float) + 3 * sizeof(double) + sizeof(uint32_t)) +
+           (uint64_t(1) << 20);
   return RBS_OK;
 }
 

@@ -103,6 +103,8 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_arm": (ctypes.c_int, [_vp, ctypes.c_uint32]),
     "rbs_armed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
     "rbs_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "rbs_state_footprint": (ctypes.c_int, [_i32, _i32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "rbs_upload": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_size_t]),
     "rbs_download": (ctypes.c_int, [_vp, _i32, _vp, ctypes.c_size_t]),
     "rbs_set_boundaries": (ctypes.c_int, [_vp, _pi32]),

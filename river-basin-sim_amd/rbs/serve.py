@@ -49,20 +49,24 @@ class Request:
     kind: str = "basin"
 
     def footprint_bytes(self) -> int:
-        """Device bytes of the state, from the layout librbs allocates (row-padded)."""
+        """Device bytes the request's state will hold, as librbs computes them for the
+        grid and the fields the request arms (rbs_state_footprint), before it exists."""
+        import ctypes
+
+        from rbs import _lib
+
         ny, nx = self.bed.shape
-        pitch = (nx + 1 + 63) // 64 * 64
-        rows = lambda r: r * pitch  # noqa: E731
-        f32 = 2 * (rows(ny) + rows(ny) + rows(ny + 1)) + 2 * rows(ny)  # h,qx,qy x2; z, n
+        arm = 0
         if self.rain is not None:
-            f32 += 2 * rows(ny)  # rain + h_comp
+            arm |= _lib.ARM_COMP | _lib.ARM_RAIN
         if self.infil is not None:
-            f32 += rows(ny)
-        f64 = 0
+            arm |= _lib.ARM_INFIL
         if self.d50 is not None:
-            f32 += 5 * rows(ny) + 2 * rows(ny + 1)
-            f64 += 2 * rows(ny)
-        return 4 * f32 + 8 * f64 + (1 << 20)
+            arm |= _lib.ARM_SED
+        fields, total = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.call("rbs_state_footprint", int(ny), int(nx), arm, ctypes.byref(fields),
+                  ctypes.byref(total))
+        return int(total.value)
 
 
 @dataclass

@@ -228,3 +228,39 @@ def test_tile_variants_are_bitwise(rbs):
     for v, o in enumerate(outs[1:], start=1):
         for name, a, b in zip(("h", "hu", "hv", "h_comp"), outs[0], o):
             assert np.array_equal(a, b), f"variant {v}: {name} differs"
+
+
+@pytest.mark.parametrize("name", ["flow_over_bump", "wet_dam_break_bumpy_slope",
+                                  "lake_over_two_bumps"])
+def test_wet_flows_match_the_reference_1d_solver(rbs, orc, name):
+    """The golden wet cases of the reference's float64 1-D solver (validation/hllc1d.py,
+    tests/golden/hllc1d_vectors.npz) on the HIP path: within the oracle's tolerance of
+    them (5e-6 m, 3e-5 m/s), and bitwise the oracle."""
+    from test_oracle_hllc import _vectors, run_1d_case
+
+    def make_g(z, dx, h0, hu0, n):
+        st = rbs.State.from_bed(z, dx, depth=h0, manning=n)
+        st.set_open_boundaries({"west": "open", "east": "open"})
+        rbs.hl.arm_hllc(st)
+        st.hu.assign(hu0)
+        return st
+
+    def make_o(z, dx, h0, hu0, n):
+        st = orc.State(z, dx, depth=h0, manning=n)
+        st.arm_hllc()
+        st.hu[...] = hu0
+        st.set_open_boundaries({"west": "open", "east": "open"})
+        return st
+
+    d = _vectors()
+    g, h_ref, u_ref = run_1d_case(lambda s, dt, t: rbs.hl.step(s, dt, t=t),
+                                  lambda s: rbs.hl.compute_dt(s, 0.45, 1e9), make_g, d, name)
+    o, _, _ = run_1d_case(lambda s, dt, t: orc.hllc_step(s, dt, t=t),
+                          lambda s: orc.hllc_compute_dt(s, 0.45, 1e9), make_o, d, name)
+    h = g.h.numpy()[0].astype(np.float64)
+    u = np.where(h > 1e-3, g.hu.numpy()[0] / np.maximum(h, 1e-3), 0.0)
+    print(f"\n[hllc vs 1-D reference, {name}] max|dh| {np.abs(h - h_ref).max():.2e} m, "
+          f"max|du| {np.abs(u - u_ref).max():.2e} m/s")
+    assert np.array_equal(g.h.numpy(), o.h) and np.array_equal(g.hu.numpy(), o.hu)
+    assert np.abs(h - h_ref).max() < 5e-6
+    assert np.abs(u - u_ref).max() < 3e-5

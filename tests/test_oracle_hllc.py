@@ -222,3 +222,67 @@ def test_dry_bed_ritter_front(orc):
     nrmse = math.sqrt(np.mean((st.h[0][inter] - h_ref[inter]) ** 2))
     assert led.max_rel_error < 1e-6
     assert nrmse < 0.05
+
+
+# --- the reference's own float64 1-D solver (validation/hllc1d.py) ---------------------
+_VEC = None
+
+
+def _vectors():
+    global _VEC
+    if _VEC is None:
+        from pathlib import Path
+
+        _VEC = np.load(Path(__file__).parent / "golden" / "hllc1d_vectors.npz")
+    return _VEC
+
+
+def test_flux_matches_the_reference_1d_flux(orc):
+    """hllc.py:191-256 against hllc1d.py:46-115 on 400 face states, dry sides included
+    (tests/golden/hllc1d_vectors.npz, written by make_hllc1d_vectors.py): the float32
+    flux within 5e-6 of the float64 one, relative to 1 + |F|."""
+    d = _vectors()
+    args = [d[f"flux/{k}"] for k in ("hL", "uL", "vL", "hR", "uR", "vR")]
+    got = np.array([orc.hllc_flux(*(float(a[i]) for a in args)) for i in range(args[0].size)])
+    for j, k in enumerate(("Fh", "Fhu", "Fhv")):
+        ref = d[f"flux/{k}"]
+        assert (np.abs(got[:, j] - ref) / (1.0 + np.abs(ref))).max() < 5e-6, k
+
+
+def run_1d_case(step, compute_dt, make_state, d, name):
+    """One of the golden wet cases on a 1 x N row with transmissive ends, stepping to
+    t_end with dt = min(CFL dt, t_end - t) as solve_1d does (hllc1d.py:197-251)."""
+    g = lambda k: d[f"{name}/{k}"]  # noqa: E731
+    nx = g("h0").size
+    st = make_state(g("z").reshape(1, nx).astype(np.float32), float(g("dx")),
+                    g("h0").reshape(1, nx).astype(np.float32),
+                    (g("h0") * g("u0")).reshape(1, nx).astype(np.float32), float(g("manning_n")))
+    t, t_end = 0.0, float(g("t_end"))
+    while t < t_end - 1e-9:
+        dt = min(compute_dt(st), t_end - t)
+        step(st, dt, t)
+        t += dt
+    return st, g("h"), g("u")
+
+
+@pytest.mark.parametrize("name", ["flow_over_bump", "wet_dam_break_bumpy_slope",
+                                  "lake_over_two_bumps"])
+def test_wet_flows_match_the_reference_1d_solver(orc, name):
+    """The 2-D float32 HLLC step on one row against the reference's float64 1-D solver:
+    flow over a bump, a wet dam break on a bumpy slope, a lake over two bumps; depth
+    within 5e-6 m and velocity within 3e-5 m/s."""
+
+    def make(z, dx, h0, hu0, n):
+        st = orc.State(z, dx, depth=h0, manning=n)
+        st.arm_hllc()
+        st.hu[...] = hu0
+        st.set_open_boundaries({"west": "open", "east": "open"})
+        return st
+
+    st, h_ref, u_ref = run_1d_case(lambda s, dt, t: orc.hllc_step(s, dt, t=t),
+                                   lambda s: orc.hllc_compute_dt(s, 0.45, 1e9), make, _vectors(), name)
+    h = st.h[0].astype(np.float64)
+    u = np.where(h > 1e-3, st.hu[0] / np.maximum(h, 1e-3), 0.0)
+    assert h.min() > 0.5  # wet everywhere: the regime both operators share
+    assert np.abs(h - h_ref).max() < 5e-6
+    assert np.abs(u - u_ref).max() < 3e-5

@@ -121,3 +121,26 @@ def test_narrow_wide_and_bedload_segments_equal_solo_results():
         got = by_id[req.rid]
         assert got.steps == alone.steps
         assert np.array_equal(got.h, alone.h), f"request {req.rid} ({req.kind}) differs"
+
+
+def test_admission_footprint_is_what_the_library_allocates():
+    """rbs_state_footprint's field bytes equal rbs_state_bytes of a fresh state of that
+    shape and arming, for every arming a request (or a HLLC or channel run) uses."""
+    import ctypes
+
+    from rbs import _lib
+    from rbs.core.state import State
+
+    _lib.load()
+    for shape in ((97, 131), (512, 2050), (300, 64)):
+        for arm in (0, _lib.ARM_COMP | _lib.ARM_RAIN | _lib.ARM_INFIL, _lib.ARM_SED,
+                    _lib.ARM_HLLC, _lib.ARM_CHAN, _lib.ARM_COMP | _lib.ARM_RAIN | _lib.ARM_SED):
+            fields, total = ctypes.c_uint64(), ctypes.c_uint64()
+            _lib.call("rbs_state_footprint", shape[0], shape[1], arm, ctypes.byref(fields),
+                      ctypes.byref(total))
+            st = State.from_bed(np.zeros(shape, np.float32), 10.0)
+            if arm:
+                st._arm(arm)
+            assert st.device_bytes() == fields.value, (shape, arm)
+            assert total.value > fields.value
+            st.close()
