@@ -128,6 +128,8 @@ def main() -> int:
     ap.add_argument("--variant", type=int, default=-1,
                     help="LI kernel variant (-1: the library's choice for the grid)")
     ap.add_argument("--tune", action="store_true", help="autotune the LI kernel variant")
+    ap.add_argument("--hllc-variant", type=int, default=-1,
+                    help="HLLC stage-kernel variant (-1: the library's default)")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only: skip the ensemble and serving figures")
     ap.add_argument("--serve-sample", type=int, default=120,
@@ -1016,6 +1018,8 @@ def bench_hllc(args) -> int:
     hg = [(0.0, 50.0), (3600.0, 800.0), (14400.0, 50.0)]
     InflowInjector([Inflow((0, j), [(t, qq / len(cells)) for t, qq in hg]) for j in cells], st)
     sched = MultiRateScheduler(end_time=14400.0, output_every=600.0, events=[3600.0])
+    if args.hllc_variant >= 0:
+        _lib.call("rbs_set_hllc_variant", st._handle, args.hllc_variant)
     run = DeviceRun(st, "hllc_fv", sched, alpha=0.45, dt_max=30.0)
     run.run_until(14400.0, max_steps=args.warmup)
     st.sync()
@@ -1049,7 +1053,8 @@ def bench_hllc(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"hllc_valley_{n}", "grid": f"{n}x{n}", "dx_m": dx,
-                   "scheme": "hllc_fv"},
+                   "scheme": "hllc_fv",
+                   "variant": args.hllc_variant if args.hllc_variant >= 0 else "default"},
         "roofline": {"bound": "hbm", "achieved": round(bytes_cell * value, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_cell * value / HBM_PEAK_GBS, 4), "traffic": None,

@@ -230,6 +230,49 @@ def test_tile_variants_are_bitwise(rbs):
             assert np.array_equal(a, b), f"variant {v}: {name} differs"
 
 
+def test_variants_are_bitwise_with_bedload_and_uniform_rain(rbs):
+    """Every variant, tiles and strips, with the bedload integral armed, uniform rain on
+    a Kahan debt, and the other edge types (closed west, open east, fixed-stage north,
+    open south): the same h, hu, hv, debts and transport integrals."""
+    import ctypes
+
+    from rbs.core import sediment
+
+    rng = np.random.default_rng(11)
+    ny, nx = 97, 131
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.008 * (nx - xx) + 0.004 * yy + 0.15 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    d0 = np.maximum(0.7 - z + 0.15 * np.cos(0.2 * xx), 0.0).astype(np.float32)
+    d50 = rng.uniform(5e-4, 3e-3, (ny, nx)).astype(np.float32)
+    count = ctypes.c_int32()
+    rbs.lib.call("rbs_hllc_variant_count", ctypes.byref(count))
+    outs, losses = [], []
+    for v in range(count.value):
+        g = rbs.State.from_bed(z, 4.0, depth=d0, manning=0.03)
+        g.set_open_boundaries({"west": "closed", "east": "open", "north": "fixed_stage",
+                               "south": "open"}, {"north": [(0.0, 0.9), (600.0, 1.1)]})
+        g.arm_source_compensation()
+        sediment.arm_sediment(g, d50, 0.4)
+        rbs.lib.call("rbs_set_hllc_variant", g._handle, v)
+        t = 0.0
+        for _ in range(60):
+            dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+            rbs.hl.step(g, dt, rain=2e-5, t=t)
+            t += dt
+        sed = g.sediment
+        outs.append((g.h.numpy(), g.hu.numpy(), g.hv.numpy(), g.h_comp.numpy(),
+                     sed.qs_int_x.numpy(), sed.qs_int_y.numpy()))
+        losses.append(g.loss_volume())
+        g.close()
+    assert np.abs(outs[0][4]).max() > 0.0 and np.abs(outs[0][1]).max() > 0.0
+    names = ("h", "hu", "hv", "h_comp", "qs_int_x", "qs_int_y")
+    for v, o in enumerate(outs[1:], start=1):
+        for name, a, b in zip(names, outs[0], o):
+            assert np.array_equal(a, b), f"variant {v}: {name} differs"
+        # the edge bank is a float64 sum whose order follows the tiling
+        assert losses[v] == pytest.approx(losses[0], rel=1e-12, abs=1e-9)
+
+
 @pytest.mark.parametrize("name", ["flow_over_bump", "wet_dam_break_bumpy_slope",
                                   "lake_over_two_bumps"])
 def test_wet_flows_match_the_reference_1d_solver(rbs, orc, name):
