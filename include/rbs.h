@@ -339,7 +339,9 @@ int rbs_li_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *fami
 int rbs_set_chan_variant(rbs_state *st, int32_t variant);
 int rbs_chan_variant_count(int32_t *count);
 int rbs_chan_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default);
-/* HLLC stage-kernel tile variants (LDS per block against halo overhead), as for LI. */
+/* HLLC stage-kernel variants, as for LI: LDS tiles (bx x by cells per block) and
+ * register strips (bx = 58 columns per wave times the waves of a block, by rows per
+ * strip; the default).  Every variant writes the same bits. */
 int rbs_set_hllc_variant(rbs_state *st, int32_t variant);
 int rbs_hllc_variant_count(int32_t *count);
 int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is_default);
