@@ -647,7 +647,21 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
       qhv = a.hv_in[o];
       qz = a.z[o];
     };
+    // Dry rows.  When rows R-4..R hold h = +0 in every lane, every face this
+    // iteration forms has both sides dry: the flux is +0 in all four components, the
+    // donor factor 1 and the limited fluxes +0, exactly as the full arithmetic gives
+    // them.  Such an iteration skips the reconstructions and fluxes and keeps only the
+    // values later iterations read (row R-1's reconstruction at its lower face and the
+    // bed-slope sources, with the signs of zero the formulas give); the update of row
+    // R-3 runs as always.  Not with the bedload integral, nor in a strip that forms a
+    // fixed-stage edge face, which can carry water into a dry cell.
+    uint32_t dry_hist = 0;
+    const int strip_c0 = strip * kHlStripW - 3;
     const int r_first = y0 - 3, r_last = y_end + 2;
+    const bool stage_edge = (bw == RBS_BC_FIXED_STAGE && strip_c0 <= 0) ||
+                            (be == RBS_BC_FIXED_STAGE && strip_c0 <= nx && nx <= strip_c0 + 63) ||
+                            (bn == RBS_BC_FIXED_STAGE && r_first <= 0) ||
+                            (bs == RBS_BC_FIXED_STAGE && r_last >= ny - 1);
     ingest(r_first);
     for (int R = r_first; R <= r_last; ++R) {
       const int it = R - r_first;
@@ -660,10 +674,23 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
         v0 = hv0 / h0;
       }
       const float e0 = h0 + z0;
+      dry_hist = (dry_hist << 1) | (__ballot(__float_as_uint(h0) != 0u) == 0 ? 1u : 0u);
+      const bool dry = !SED && !stage_edge && (dry_hist & 0x1Fu) == 0x1Fu;  // wave-uniform
 
       // row R-1 along y; the face between rows R-2 and R-1 (normal v, transverse u)
       float yh1 = 0.0f, yl1 = 0.0f, yr1 = 0.0f, yt1 = 0.0f, sy1 = 0.0f, dy1 = 0.0f;
-      if (it >= 2) {  // wave-uniform, as every branch on `it` and R below
+      if (dry) {  // the face is +0; row R-1's values at its lower face, as hl_recon forms them
+        const float hw = 0.5f * 0.0f;
+        const float me = minmod(e1 - e2, e0 - e1), mz = minmod(z1 - z2, z0 - z1);
+        pe = e1 + hw * me;
+        pz = z1 + hw * mz;
+        pn = v1 + hw * minmod(v1 - v2, v0 - v1);
+        pt = u1 + hw * minmod(u1 - u2, u0 - u1);
+        ph = fmx(pe - pz, 0.0f);
+        const float hm = fmx((e1 - hw * me) - (z1 - hw * mz), 0.0f);
+        const float g0 = kG;
+        sy1 = -g0 * 0.5f * (ph + hm) * (0.0f * mz);
+      } else if (it >= 2) {  // wave-uniform, as every branch on `it` and R below
         const HlRecon ry = hl_recon(h1, e1, z1, v1, u1, h2, e2, z2, v2, u2, h0, e0, z0, v0, u0);
         sy1 = ry.src;
         if (it >= 3) {
@@ -687,7 +714,16 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
       // row R-2 along x; its x-faces
       float xh = 0.0f, xl = 0.0f, xr = 0.0f, xt = 0.0f, sx = 0.0f, dxf = 0.0f;
       float b2 = 1.0f, xq = 0.0f, yq = 0.0f;
-      if (it >= 4) {
+      if (dry) {  // row R-2's faces are +0, its donor factor 1; its x bed-slope source
+        const float zl_ = hl_left(z2), zr_ = hl_right(z2), el_ = hl_left(e2), er_ = hl_right(e2);
+        const float hw = 0.5f * 0.0f;
+        const float me = minmod(e2 - el_, er_ - e2), mz = minmod(z2 - zl_, zr_ - z2);
+        const float hp = fmx((e2 + hw * me) - (z2 + hw * mz), 0.0f);
+        const float hm = fmx((e2 - hw * me) - (z2 - hw * mz), 0.0f);
+        const float g0 = kG;
+        sx = -g0 * 0.5f * (hp + hm) * (0.0f * mz);
+        yq = yh2;
+      } else if (it >= 4) {
         const float hl_ = hl_left(h2), el_ = hl_left(e2), zl_ = hl_left(z2);
         const float ul_ = hl_left(u2), vl_ = hl_left(v2);
         const float hr_ = hl_right(h2), er_ = hl_right(e2), zr_ = hl_right(z2);

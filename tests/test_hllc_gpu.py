@@ -273,6 +273,65 @@ def test_variants_are_bitwise_with_bedload_and_uniform_rain(rbs):
         assert losses[v] == pytest.approx(losses[0], rel=1e-12, abs=1e-9)
 
 
+def test_dry_floodplain_variants_and_oracle_are_bitwise(rbs, orc):
+    """A channel in a dry floodplain without rain, so that whole rows of a strip stay
+    exactly dry (h = +0) and its iterations take the dry path; a patch of the bed at
+    -0.0; a fixed-stage edge (north) that can wet a dry cell, and a closed and two open
+    edges.  Every variant writes the same bits, and the default equals the oracle."""
+    import ctypes
+
+    rng = np.random.default_rng(21)
+    ny, nx = 150, 301
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.004 * np.abs(xx - 150) + 0.0005 * (ny - yy) + 0.02 * rng.uniform(0, 1, (ny, nx)))
+    z = z.astype(np.float32)
+    z[100:112, 240:262] = np.float32(-0.0)
+    assert np.signbit(z[105, 250])
+    d0 = np.where(np.abs(xx - 150) < 12, 0.6, 0.0).astype(np.float32)
+    bnd = {"west": "closed", "east": "open", "north": "fixed_stage", "south": "open"}
+    curve = {"north": [(0.0, 0.45), (60.0, 0.8)]}
+
+    def run(v=None):
+        g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+        g.set_open_boundaries(bnd, curve)
+        if v is not None:
+            rbs.lib.call("rbs_set_hllc_variant", g._handle, v)
+        t = 0.0
+        for _ in range(80):
+            dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+            rbs.hl.step(g, dt, t=t)
+            t += dt
+        out = (g.h.numpy(), g.hu.numpy(), g.hv.numpy())
+        g.close()
+        return out, t
+
+    count = ctypes.c_int32()
+    rbs.lib.call("rbs_hllc_variant_count", ctypes.byref(count))
+    outs = [run(v)[0] for v in range(count.value)]
+    for v, o in enumerate(outs[1:], start=1):
+        for name, a, b in zip(("h", "hu", "hv"), outs[0], o):
+            assert np.array_equal(a, b), f"variant {v}: {name} differs"
+    # the run keeps a dry floodplain and wets the north edge through the stage
+    h = outs[0][0]
+    assert (h == 0.0).mean() > 0.5 and h[0].max() > 0.0
+    # the default variant against the oracle, step for step
+    o = orc.State(z, 5.0, depth=d0, manning=0.03)
+    o.arm_hllc()
+    o.set_open_boundaries(bnd, curve)
+    g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+    g.set_open_boundaries(bnd, curve)
+    t = 0.0
+    for _ in range(80):
+        dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+        assert abs(dt - orc.hllc_compute_dt(o, 0.45, 2.0)) <= 1e-6 * dt
+        rbs.hl.step(g, dt, t=t)
+        orc.hllc_step(o, dt, t=t)
+        t += dt
+    for name, a, b in (("h", g.h.numpy(), o.h), ("hu", g.hu.numpy(), o.hu), ("hv", g.hv.numpy(), o.hv)):
+        assert np.array_equal(a, b), f"oracle: {name} differs"
+    g.close()
+
+
 @pytest.mark.parametrize("name", ["flow_over_bump", "wet_dam_break_bumpy_slope",
                                   "lake_over_two_bumps"])
 def test_wet_flows_match_the_reference_1d_solver(rbs, orc, name):
