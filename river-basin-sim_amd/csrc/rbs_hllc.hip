@@ -34,6 +34,7 @@ struct HlArgs {
   const float *rain, *infil;
   const float *d50;
   float *qs_int_x, *qs_comp_x, *qs_int_y, *qs_comp_y;
+  uint32_t *hquiet;  // per block: U1 (bit 0) and U^n momentum (bit 1) left all +0
   const DevCtl *ctl;
   Partials part;
   double dx64;
@@ -624,8 +625,68 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
   const bool bank_w = loss && bw != RBS_BC_CLOSED, bank_e = loss && be != RBS_BC_CLOSED;
   const bool bank_n = loss && bn != RBS_BC_CLOSED, bank_s = loss && bs != RBS_BC_CLOSED;
 
+  // Dry blocks.  A block whose stage inputs hold h = +0 over all its rows and
+  // columns, three out (for stage 2 also U^n's depth at its own cells), forms only +0
+  // faces, and its update writes +0 to every output: the depths and momenta of U1 in
+  // stage 1, of U^n in stage 2 (no rain, no Kahan debt to fold).  When the block's flag
+  // says that its outputs already hold +0 (set by the last launch that computed the
+  // block and wrote only +0; uploads of depth or momentum and a change of tiling clear
+  // the flags; point sources and reservoirs write depths, which the check reads), the
+  // block writes nothing but its partials, which are then +0 too.  Not with the bedload
+  // integral, nor near a fixed-stage edge.
+  const uint32_t qbit = STAGE == 1 ? 1u : 2u;
+  {
+    const int bx0 = tx * NW * kHlStripW - 3, bx1 = bx0 + NW * kHlStripW + 6;
+    const bool near_stage = (bw == RBS_BC_FIXED_STAGE && bx0 <= 0) ||
+                            (be == RBS_BC_FIXED_STAGE && bx0 <= nx && nx < bx1) ||
+                            (bn == RBS_BC_FIXED_STAGE && y0 <= 4) ||
+                            (bs == RBS_BC_FIXED_STAGE && y_end >= ny - 4);
+    const bool no_sources = STAGE == 1 || (ctl->rain == 0.0f && !(a.flags & HL_COMP) &&
+                                           (!(a.flags & HL_FIELD) || ctl->rain_scale == 0.0f));
+    if (!SED && no_sources && !near_stage && (a.hquiet[tile] & qbit) != 0u) {  // block-uniform
+      uint32_t wet = 0;
+      if (live) {  // eight loads in flight per round
+        for (int R0 = y0 - 3; R0 <= y_end + 2; R0 += 8) {
+          uint32_t w8[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int R = R0 + k <= y_end + 2 ? R0 + k : y_end + 2;
+            w8[k] = __float_as_uint(a.h_in[(int64_t)clampi(R, 0, ny - 1) * P + cc]);
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) wet |= w8[k];
+        }
+        if (STAGE == 2 && own)
+          for (int i0 = y0; i0 < y_end; i0 += 8) {
+            uint32_t w8[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const int i = i0 + k < y_end ? i0 + k : y_end - 1;
+              w8[k] = __float_as_uint(a.hn[(int64_t)i * P + c]);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) wet |= w8[k];
+          }
+      }
+      if (!__syncthreads_or(wet != 0u)) {
+        if (threadIdx.x == 0) {
+          if (STAGE == 1) {
+            a.part.edge[tile] = 0.0;
+          } else {
+            a.part.speed_max[tile] = 0.0f;
+            a.part.vol[tile] = 0.0;
+            a.part.infil[tile] = 0.0;
+            a.part.edge[tile] = a.part.edge[tile] + 0.0;
+          }
+        }
+        return;
+      }
+    }
+  }
+
   float m_s = 0.0f;
   double s_vol = 0.0, s_inf = 0.0, bank = 0.0;
+  uint32_t out_bits = 0;  // what the block wrote to the outputs the flag vouches for
   if (live) {
     // the window (rows relative to the newest, R)
     float h1 = 0.0f, e1 = 0.0f, z1 = 0.0f, u1 = 0.0f, v1 = 0.0f, hu1 = 0.0f, hv1 = 0.0f;
@@ -809,6 +870,7 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
           a.h_out[o] = hh;
           a.hu_out[o] = hu_1;
           a.hv_out[o] = hv_1;
+          out_bits |= __float_as_uint(hh) | __float_as_uint(hu_1) | __float_as_uint(hv_1);
         } else {  // _rk_stage2, in place on U^n
           float hh = fmx(0.5f * (a.hn[o] + hg + dt * dh), 0.0f);
           float hu = 0.0f, hv = 0.0f;
@@ -856,6 +918,7 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
           a.h_out[o] = hh;
           a.hu_out[o] = hu;
           a.hv_out[o] = hv;
+          out_bits |= __float_as_uint(hu) | __float_as_uint(hv);
           // _reduce_wavespeed (:948-966) of the new state
           const float cw = sqrtf(g * fmx(hh, 0.0f));
           float sp = cw;
@@ -903,8 +966,10 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
       dy2 = dy1;
     }
   }
+  const bool quiet_out = !__syncthreads_or(out_bits != 0u);
   block_reduce4<64 * NW>(m_s, s_vol, s_inf, bank, s_red);
   if (threadIdx.x == 0) {
+    a.hquiet[tile] = (a.hquiet[tile] & ~qbit) | (quiet_out ? qbit : 0u);
     if (STAGE == 1) {
       a.part.edge[tile] = bank;
     } else {
@@ -938,9 +1003,10 @@ static const HlVariant kHlVariants[] = {
     RBS_HL_STRIP(1, 32),  RBS_HL_STRIP(1, 16),  RBS_HL_STRIP(2, 32),  RBS_HL_STRIP(1, 128),
 };
 constexpr int kNumHlVariants = sizeof(kHlVariants) / sizeof(kHlVariants[0]);
-// one 58-column strip per block, 32 rows: 1349 us per step at 8192^2 against 2395 us
-// for the fastest LDS tile (16x16); 64 rows 1482 us, 16 rows 1354 us
-constexpr int kDefaultHlVariant = 12;
+// one 58-column strip per block, 16 rows: on the 8192^2 valley (mostly dry) 317 us per
+// step against 390 us for 32 rows (the finer blocks skip more dry area) and 2388 us for
+// the fastest LDS tile (16x16); computing every cell, 16 and 32 rows ran alike
+constexpr int kDefaultHlVariant = 13;
 
 int hllc_variant_count() { return kNumHlVariants; }
 int hllc_default_variant() { return kDefaultHlVariant; }
@@ -977,6 +1043,7 @@ int hllc_launch_step(rbs_state *s, bool, double, double, double, double) {
   a.qs_comp_x = s->qs_comp_x;
   a.qs_int_y = s->qs_int_y;
   a.qs_comp_y = s->qs_comp_y;
+  a.hquiet = s->hquiet;
   a.ctl = s->ctl;
   a.part = s->part;
   a.dx64 = s->dx;

@@ -27,6 +27,10 @@ constexpr double kHDry64 = 1.0e-3;
 constexpr double kEpsT = 1e-6;
 
 constexpr int kThreads = 256;  // block size of every tiled kernel
+// Block partials beyond kFoldMin are folded in two levels: kFoldBlocks blocks first,
+// then the one-block finalize over their kFoldBlocks results.
+constexpr int kFoldBlocks = 128;
+constexpr int kFoldMin = 4096;
 
 // ---- threads ----------------------------------------------------------------
 // Distinct handles may be driven from distinct host threads.  A stream capture
@@ -186,9 +190,14 @@ struct rbs_state {
   // reductions
   rbs::Partials part{};
   int32_t part_cap = 0;
+  // the first level of a two-level fold of many block partials (kFoldBlocks entries)
+  rbs::Partials fold{};
   // per LI tile: which buffers and compensations hold only +0 (the strip kernels
   // skip dry tiles with it); part_cap words, cleared by quiet_reset
   uint32_t *quiet = nullptr;
+  // per HLLC block: bit 0 when stage 1 left the block's U1 cells all +0, bit 1 when
+  // stage 2 left their U^n momentum all +0 (the strips skip dry blocks with it)
+  uint32_t *hquiet = nullptr;
   rbs::DevCtl *ctl = nullptr;        // device
   rbs::DevCtl *ctl_host = nullptr;   // pinned host mirror for status reads
   double *d_sync = nullptr;

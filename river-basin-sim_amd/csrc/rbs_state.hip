@@ -366,7 +366,8 @@ static int state_destroy(rbs_state *s) {
                   s->h_comp, s->rain, s->infil, s->hu, s->hv, s->h1, s->hu1, s->hv1,
                   s->d50, s->qs_int_x, s->qs_comp_x, s->qs_int_y, s->qs_comp_y, s->z0,
                   s->dz_lo, s->dz_hi, s->dz_cum, s->dz_unapplied, s->sed_part, s->part.speed_max,
-                  s->part.vol, s->part.infil, s->part.edge, s->quiet, s->ctl, s->d_sync,
+                  s->part.vol, s->part.infil, s->part.edge, s->quiet, s->hquiet, s->fold.speed_max, s->fold.vol,
+                  s->fold.infil, s->fold.edge, s->ctl, s->d_sync,
                   s->inflow.d_rows, s->inflow.d_cols, s->inflow.d_off, s->inflow.d_kt,
                   s->inflow.d_kq, s->inflow.d_comp, s->inflow.d_vol, s->ch_w, s->ch_d,
                   s->ch_n, s->qx_ch[0], s->qx_ch[1], s->qx_fp[0], s->qx_fp[1], s->qy_ch[0],
@@ -411,9 +412,9 @@ int rbs_state_footprint(int32_t ny, int32_t nx, uint32_t arm, uint64_t *fields,
   probe.chan_variant = chan_default_variant();
   const uint64_t nblk = (uint64_t)max_blocks(&probe);
   *fields = field_bytes(ny, probe.pitch, arm);
-  // partials: a float, three doubles and a dry-tile word per block; then the control
-  // block, point-source tables, reduction scratch and staging the state adds later
-  *total = *fields + nblk * (sizeof(float) + 3 * sizeof(double) + sizeof(uint32_t)) +
+  // partials: a float, three doubles and two dry-block words per block; then the
+  // control block, point-source tables, reduction scratch and staging the state adds
+  *total = *fields + nblk * (sizeof(float) + 3 * sizeof(double) + 2 * sizeof(uint32_t)) +
            (uint64_t(1) << 20);
   return RBS_OK;
 }
@@ -500,7 +501,8 @@ int rbs_upload(rbs_state *s, int32_t field, const void *host, size_t bytes) {
   rc = copy_rows(s, v, const_cast<void *>(host), true);
   if (rc) return rc;
   if (field == RBS_F_H || field == RBS_F_QX || field == RBS_F_QY || field == RBS_F_H_COMP ||
-      field == RBS_F_QS_COMP_X || field == RBS_F_QS_COMP_Y) {
+      field == RBS_F_QS_COMP_X || field == RBS_F_QS_COMP_Y || field == RBS_F_HU ||
+      field == RBS_F_HV) {
     rc = quiet_reset(s, s->stream);
     if (rc) return rc;
   }
@@ -828,9 +830,14 @@ int rbs_li_variant_count(int32_t *count) {
 }
 
 int rbs_set_hllc_variant(rbs_state *s, int32_t variant) {
+  RBS_API_SHARED;
   if (!s) return set_error("null state"), RBS_E_ARG;
   if (variant < 0 || variant >= hllc_variant_count())
     return set_error("HLLC variant %d is not in [0, %d)", variant, hllc_variant_count()), RBS_E_ARG;
+  if (variant != s->hllc_variant) {
+    const int rc = quiet_reset(s, s->stream);  // another tiling
+    if (rc) return rc;
+  }
   s->hllc_variant = variant;
   invalidate_graph(s);  // a captured device-driven run launches the old tiles
   return RBS_OK;

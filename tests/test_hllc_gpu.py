@@ -332,6 +332,79 @@ def test_dry_floodplain_variants_and_oracle_are_bitwise(rbs, orc):
     g.close()
 
 
+def test_dry_blocks_wake_for_sources_and_uploads(rbs):
+    """Blocks that were dry and skipped their stages come back when water reaches them:
+    a point source that starts into a dry field, an upload of depth into a dry area
+    and an upload of momentum onto dry cells (which the next step zeroes) mid-run.
+    The default strips, which skip dry blocks, against the LDS tiles, which compute
+    every cell: the same bits, step for step at the end of each phase."""
+    from rbs.processes.inflow import Inflow, InflowInjector
+
+    rng = np.random.default_rng(5)
+    ny, nx = 130, 257
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    z = (0.003 * np.abs(xx - 60) + 0.02 * rng.uniform(0, 1, (ny, nx))).astype(np.float32)
+    d0 = np.where(np.abs(xx - 60) < 6, 0.5, 0.0).astype(np.float32)
+    blob = np.zeros((ny, nx), np.float32)
+    blob[90:100, 200:215] = 0.4
+    mom = np.zeros((ny, nx), np.float32)
+    mom[20:40, 150:250] = 0.3
+
+    def run(variant):
+        g = rbs.State.from_bed(z, 5.0, depth=d0, manning=0.03)
+        g.set_open_boundaries({"east": "open"})
+        if variant is not None:
+            rbs.lib.call("rbs_set_hllc_variant", g._handle, variant)
+        inj = InflowInjector([Inflow((60, 180), [(0.0, 0.0), (30.0, 0.0), (31.0, 4.0),
+                                                 (1e9, 4.0)])], g)
+        snaps, t = [], 0.0
+        for k in range(90):
+            if k == 40:
+                g.h.assign(g.h.numpy() + blob)
+                g.hu.assign(np.where(g.h.numpy() == 0.0, mom, g.hu.numpy()))
+            dt = rbs.hl.compute_dt(g, 0.45, 2.0)
+            inj.apply(g, t, dt)
+            rbs.hl.step(g, dt, t=t)
+            t += dt
+            if k in (39, 41, 89):
+                snaps.append((g.h.numpy(), g.hu.numpy(), g.hv.numpy()))
+        g.close()
+        return snaps, t
+
+    strips, t_s = run(None)
+    tiles, t_t = run(4)
+    assert t_s == t_t
+    for phase, (a, b) in enumerate(zip(strips, tiles)):
+        for name, x, y in zip(("h", "hu", "hv"), a, b):
+            assert np.array_equal(x, y), f"phase {phase}: {name} differs"
+    h_end = strips[-1][0]
+    assert h_end[60, 180] > 0.0 and h_end[95, 207] > 0.0  # both new waters are there
+    assert (h_end == 0.0).mean() > 0.3  # and much of the field is still dry
+
+
+def test_many_blocks_fold_in_two_levels(rbs):
+    """A 2048^2 HLLC state has 4608 strip blocks, past the point where the block
+    partials are folded in two levels: the ledger's volume equals the float64 sum of
+    the depths, and the wave-speed maximum the float32 formula over the whole grid."""
+    rng = np.random.default_rng(3)
+    n, dx = 2048, 5.0
+    z = (0.5 * rng.uniform(0, 1, (n, n))).astype(np.float32)
+    h0 = np.maximum(1.2 - z + 0.2 * rng.uniform(0, 1, (n, n)), 0.0).astype(np.float32)
+    g = rbs.State.from_bed(z, dx, depth=h0, manning=0.03)
+    for _ in range(3):
+        rbs.hl.step(g, rbs.hl.compute_dt(g, 0.45, 2.0))
+    led = g.ledger()
+    h, hu, hv = g.h.numpy(), g.hu.numpy(), g.hv.numpy()
+    vol = float(h.astype(np.float64).sum()) * dx * dx
+    assert led.volume == pytest.approx(vol, rel=1e-12)
+    c = np.sqrt(np.float32(9.81) * np.maximum(h, np.float32(0.0)))
+    wet = h > np.float32(1e-3)
+    hs = np.where(wet, h, np.float32(1.0))
+    sp = np.where(wet, np.maximum(np.abs(hu) / hs, np.abs(hv) / hs) + c, c)
+    assert np.float32(led.speed_max) == sp.max()
+    g.close()
+
+
 @pytest.mark.parametrize("name", ["flow_over_bump", "wet_dam_break_bumpy_slope",
                                   "lake_over_two_bumps"])
 def test_wet_flows_match_the_reference_1d_solver(rbs, orc, name):
