@@ -1039,6 +1039,7 @@ def bench_hllc(args) -> int:
     log(f"[hllc] stage kernels {step_us:.0f} us per step (HIP events)")
     sweep = hllc_sweep(st, dt_k, n * n) if args.sweep else None
     st.close()
+    wet_sweep = hllc_wet_sweep(bed, dx) if args.sweep else None
     early = {"steps": taken, "wall_s": round(el, 3), "gcell_s": round(value, 3),
              "t_end_s": round(led.t, 1), "stage_kernels_us_per_step": round(step_us, 1)}
     flood = hllc_flood(args, bed, depth, dx, stage, cells, hg) if args.hllc_hours > 0 else None
@@ -1062,18 +1063,50 @@ def bench_hllc(args) -> int:
         "config": {"workload": f"hllc_valley_{n}", "grid": f"{n}x{n}", "dx_m": dx,
                    "scheme": "hllc_fv",
                    "variant": args.hllc_variant if args.hllc_variant >= 0 else "default"},
-        "roofline": {"bound": "hbm", "achieved": round(bytes_cell * value, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(bytes_cell * value / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": bytes_cell},
+        "roofline": hllc_roofline(n, args.hllc_variant, step_us, bytes_cell),
         "lake_at_rest": lake,
         "early_window": early,
         "flood": flood,
     }
     if sweep:
         out["variant_sweep"] = sweep
+        out["variant_sweep_wet"] = wet_sweep
     print(json.dumps(out), flush=True)
     return 0
+
+
+def hllc_roofline(n: int, variant: int, step_us: float, bytes_cell: int) -> dict:
+    """The stage kernels' HBM rate: the measured bytes per step of the library's strips
+    on the 8192^2 valley (FETCH_SIZE and WRITE_SIZE, calibrated on a 1 GiB copy) over
+    their HIP-event time in the early window.  Cell-updates x bytes per cell would not
+    do: blocks that are dry and unchanged write nothing and read only their depths."""
+    out = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": None, "traffic": None, "kernel_us": round(step_us, 1),
+           "bytes_per_computed_cell": bytes_cell}
+    tf = ROOT / "profiles" / "hllc_valley_8192_traffic.json"
+    if n == 8192 and variant < 0 and tf.exists():
+        traffic = json.loads(tf.read_text())["bytes_per_step"]
+        achieved = traffic / (step_us * 1e-6) / 1e9
+        out.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4),
+                   traffic=traffic)
+    return out
+
+
+def hllc_wet_sweep(bed, dx: float) -> list:
+    """Every HLLC variant on the valley's bed under a lake 1 m above its highest point:
+    every cell wet, every face computed (the dry paths never apply)."""
+    from rbs import _lib
+    from rbs.core import hllc
+    from rbs.core.state import State
+
+    eta = float(bed.max()) + 1.0
+    st = State.from_bed(bed, dx, depth=(eta - bed.astype(np.float64)).astype(np.float32),
+                        manning=0.035)
+    dt = hllc.compute_dt(st, 0.45, 30.0)
+    log(f"[hllc] wet sweep: lake at {eta:.1f} m over the {bed.shape[0]}^2 valley, dt {dt:.3f} s")
+    rows = hllc_sweep(st, dt, bed.size)
+    st.close()
+    return rows
 
 
 def dry_block_fraction(h, rows: int = 16, cols: int = 58, halo: int = 3) -> float:
