@@ -107,12 +107,147 @@ __device__ __forceinline__ float pow_third(float x) {
   if (SUB && tiny) p *= (P == 7 ? 0x1p-112 : P == 4 ? 0x1p-64 : 0x1p-16);
   return (float)p;
 }
+// The same correctly rounded power, by a shorter route that is exact where it claims to
+// be and hands every other input to pow_third.  For x in [2^-11, 2^20]:
+//  1. L ~ log2 x (v_log_f32) and y ~ x^(-1/3) = 2^(-L/3) (v_exp_f32), good to about
+//     2^-21; r = 1 - x y^3 exactly as in pow_third, so the series of (1 - r)^(-2/3)
+//     needs r and r^2 only (the r^3 term is below 2^-58);
+//  2. u = d ln x = (d ln 2) L in float32, good to 2.5e-13 absolute (|u| < 1.1e-6);
+//  3. x^q y^2 (1 + g) in float64 as in pow_third, now good to 2^-40.9 relative: within
+//     2^11.1 float64 ulps of the exact power.
+// Its float32 rounding is therefore the exact power's unless the float64 value lies
+// within 2^13 float64 ulps of a float32 rounding midpoint (the low 29 significand bits
+// within 2^13 of 2^28): such inputs (about 3 in 10^5), and every x outside the range
+// ([2^-10, 2^20] for 4/3 and 1/3, whose radii can be tiny; above 2^20 for 7/3, whose
+// depths are at least H_DRY), take pow_third.  Checked on the device for every float32
+// of [2^-11, 2^21] and each exponent (rbs_check_arithmetic): no input where the short
+// route claims exactness and differs from pow_third.
+template <int P>
+__device__ __forceinline__ float pow_third_hw(float x, bool &exact) {
+  constexpr double d = ((double)P - 3.0 * (double)(float)((double)P / 3.0)) / 3.0;
+  constexpr float ufac = (float)(d * 0.6931471805599453);
+  constexpr uint32_t kMargin = 1u << 13;
+  const float L = __builtin_amdgcn_logf(x);
+  const float yb = __builtin_amdgcn_exp2f(L * (-1.0f / 3.0f));
+  const float uf = L * ufac;
+  const double a = (double)x, b = (double)yb;
+  const double b2 = b * b;
+  const double r = fma(-(a * b), b2, 1.0);
+  const double s2 = r * fma(r, 5.0 / 9.0, 2.0 / 3.0);
+  const double u = (double)uf;
+  const double g = fma(-u, fma(-0.5, u, s2) + 1.0, s2);
+  // x^q y^2 from the exact x y: (x y)^2 x, (x y)^2, x y^2
+  const double ab = a * b;
+  double A;
+  if (P == 7) A = (ab * ab) * a;
+  else if (P == 4) A = ab * ab;
+  else A = a * b2;
+  const double p = fma(A, g, A);
+  const uint32_t lo = (uint32_t)__double_as_longlong(p) & 0x1fffffffu;
+  // (the friction power's callers pass depths of at least H_DRY > 2^-10)
+  exact = (lo - (0x10000000u - kMargin)) > 2u * kMargin && x <= 0x1p20f &&
+          (P == 7 || x >= 0x1p-10f);
+  return (float)p;
+}
+template <int P, bool SUB>
+__device__ __forceinline__ float pow_third_fast(float x) {
+  bool exact;
+  float p = pow_third_hw<P>(x, exact);
+  if (!exact) p = pow_third<P, SUB>(x);  // a divergent branch, skipped by most waves
+  return p;
+}
 // wp.pow(h, 7.0/3.0) for a depth h >= H_DRY (friction.py:31)
-__device__ __forceinline__ float pow_e73(float x) { return pow_third<7, false>(x); }
+__device__ __forceinline__ float pow_e73(float x) { return pow_third_fast<7, false>(x); }
 // wp.pow(R, 4.0/3.0) and wp.pow(R, 1.0/3.0) of a hydraulic radius (friction.py:57,
 // sediment.py:168), which a narrow channel can make arbitrarily small
-__device__ __forceinline__ float pow_e43(float x) { return pow_third<4, true>(x); }
-__device__ __forceinline__ float pow_e13(float x) { return pow_third<1, true>(x); }
+__device__ __forceinline__ float pow_e43(float x) { return pow_third_fast<4, true>(x); }
+__device__ __forceinline__ float pow_e13(float x) { return pow_third_fast<1, true>(x); }
+
+// a / b correctly rounded, by the reciprocal, one Newton step and two Markstein
+// corrections (every remainder exact in an FMA): v_rcp_f32 scales exactly by powers of
+// two over 2^-100..2^100, so the result is the IEEE quotient whenever no step leaves
+// the normal range -- |a| in [2^-100, 2^100], |b| in [2^-100, 2^100], a/b normal with
+// room for the remainders.  Checked on the device against the IEEE division on every
+// pair of float32 significands (7.0e13 pairs), and on 2.1e9 random pairs of the ranges
+// the callers use.  Callers check their operands' ranges; the compiler's division (two
+// v_div_scale, v_div_fmas, v_div_fixup around the same steps) stays for the rest.
+__device__ __forceinline__ float div_rn(float a, float b) {
+  float y = __builtin_amdgcn_rcpf(b);
+  const float e = __builtin_fmaf(-b, y, 1.0f);
+  y = __builtin_fmaf(e, y, y);
+  float q = a * y;
+  float r = __builtin_fmaf(-b, q, a);
+  q = __builtin_fmaf(r, y, q);
+  r = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(r, y, q);
+}
+
+// The rare operands outside div_rn's ranges take the IEEE division in a branch on the
+// whole wave's vote, which the compiler keeps as a skipped scalar branch (a per-lane
+// condition it would evaluate speculatively on every lane).  The vote is one ballot of
+// the lanes that need the long route.
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+// Pins a value as computed on every lane: the short quotients are cheaper than the
+// branch the compiler would otherwise wrap around the one arm of a select that uses them.
+__device__ __forceinline__ float opaque(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// 1 + c / p, each operation rounded as float32, for c >= 0 (a friction numerator) and a
+// positive power p.  c / p by div_rn for c in [2^-50, 2^45] and p in [2^-24, 2^47]
+// (quotients in [2^-97, 2^69]); for c below 2^-50 and p >= 2^-24 the quotient is below
+// 2^-26, and 1 + c / p rounds to 1 exactly as 1 + 0 does; the IEEE division otherwise.
+__device__ __forceinline__ float one_plus_ratio(float c, float p) {
+  const bool p_ok = p >= 0x1p-24f && p <= 0x1p47f;
+  const bool fast = (c <= 0x1p45f && p_ok) || (c < 0x1p-50f && p >= 0x1p-24f);
+  const float xd = opaque(div_rn(c, p));
+  float x = c < 0x1p-50f ? 0.0f : xd;
+  if (!wave_all(fast)) x = fast ? x : c / p;
+  return 1.0f + x;
+}
+// The Manning form for a depth h >= H_DRY: its power p = h^float32(7/3) is then at
+// least 2^-23.3, and at most 2^47 while h <= 2^20.
+__device__ __forceinline__ float one_plus_ratio_h(float c, float p, float h) {
+  const bool fast = c <= 0x1p45f && h <= 0x1p20f;
+  const float xd = opaque(div_rn(c, p));
+  float x = c < 0x1p-50f ? 0.0f : xd;
+  if (!wave_all(fast)) x = fast ? x : c / p;
+  return 1.0f + x;
+}
+
+// a / den for a Manning denominator den = 1 + x, x >= 0 (or NaN): div_rn for |a| in
+// [2^-96, 2^96] and den up to 2^24 (quotients in [2^-120, 2^96]); a zero keeps its sign
+// (den > 0), as the IEEE quotient does; the IEEE division otherwise.
+__device__ __forceinline__ bool den_fast(float a, float den) {
+  const float m = fabsf(a);
+  return den <= 0x1p24f && ((m >= 0x1p-96f && m <= 0x1p96f) || m == 0.0f);
+}
+__device__ __forceinline__ float div_by_den_short(float a, float den) {
+  const float q = opaque(div_rn(a, den));
+  return fabsf(a) == 0.0f ? a : q;
+}
+__device__ __forceinline__ float div_by_den(float a, float den) {
+  const bool fast = den_fast(a, den);
+  float q = div_by_den_short(a, den);
+  if (!wave_all(fast)) q = fast ? q : a / den;
+  return q;
+}
+
+// The Manning face update num / (1 + c / h^float32(7/3)) (local_inertial.py:72-129 with
+// friction.py:18-31) for a depth h >= H_DRY, every operation rounded as float32 in the
+// reference's order: both quotients by the short route, one vote for the wave, and the
+// IEEE divisions for the lanes outside the short route's ranges.
+__device__ __forceinline__ float manning_update(float num, float c, float h) {
+  const float p = pow_e73(h);
+  const float xd = opaque(div_rn(c, p));
+  const float x = c < 0x1p-50f ? 0.0f : xd;
+  const float den = 1.0f + x;
+  float q = div_by_den_short(num, den);
+  const bool fast = c <= 0x1p45f && h <= 0x1p20f && den_fast(num, den);
+  if (!wave_all(fast)) q = fast ? q : num / (1.0f + c / p);
+  return q;
+}
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   const double a = (double)x;
   return (float)(a * sqrt(a));
@@ -120,8 +255,9 @@ __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
 
 // friction.py:18-31 manning_denominator, 1 + g dt n^2 |q| / h^e, e = float32(7/3), in
 // the reference's float32 order: the product left to right, one division, one sum.
+// h_flow >= H_DRY (every caller's wet test).
 __device__ __forceinline__ float manning_den(float q_prev, float h_flow, float n, float g, float dt) {
-  return 1.0f + g * dt * n * n * fabsf(q_prev) / pow_e73(h_flow);
+  return one_plus_ratio_h(g * dt * n * n * fabsf(q_prev), pow_e73(h_flow), h_flow);
 }
 
 // sources.py:66-82 kahan_add
@@ -152,7 +288,7 @@ __device__ __forceinline__ float div_uniform(float a, float b, float rb) {
   const float q1 = __builtin_fmaf(r, rb, q0);
   const float m = fabsf(a);
   const bool normal = m >= 0x1p-100f && m <= 0x1p+100f;
-  if (__all(normal)) return q1;
+  if (wave_all(normal)) return q1;
   return normal ? q1 : a / b;
 }
 
@@ -166,8 +302,7 @@ __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, floa
   float n_face = 0.5f * (n_a + n_b);
   float slope = div_uniform(eta_b - eta_a, dx, rdx);
   float num = q - g * h_flow * dt * slope;
-  float den = manning_den(q, h_flow, n_face, g, dt);
-  return num / den;
+  return manning_update(num, g * dt * n_face * n_face * fabsf(q), h_flow);  // manning_den
 }
 
 // ---- sub-grid channels (channels.py:59-138, local_inertial.py:266-500) -------------
@@ -222,7 +357,7 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
   const float h_fp = eta_max - z_bank;
   if (!(h_fp < kHDry)) {
     const float num = q_fp - g * h_fp * dt * slope;
-    fp = num / manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt);
+    fp = div_by_den(num, manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt));
   }
   const float w_face = fmn(a.w, b.w);
   float ch = 0.0f;
@@ -233,8 +368,8 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
       const float radius = w_face * h_ch / (w_face + 2.0f * h_ch);
       const float nc = 0.5f * (a.nc + b.nc);
       const float num = q_ch - g * h_ch * dt * slope;
-      const float den = 1.0f + g * dt * nc * nc * fabsf(q_ch) / (h_ch * pow_e43(radius));
-      ch = num / den;
+      const float den = one_plus_ratio(g * dt * nc * nc * fabsf(q_ch), h_ch * pow_e43(radius));
+      ch = div_by_den(num, den);
     }
   }
   q_fp = fp;
