@@ -363,19 +363,19 @@ int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
                     int64_t *graph_launches, double *gpu_ms);
 
 /* ---- self-check of the device arithmetic (no reference counterpart) ------------------ */
-/* The kernels evaluate the reference's float32 powers (friction.py:31, :57,
- * sediment.py:168) and the Manning divisions correctly rounded, through shorter routes
+/* The local-inertial face evaluates the reference's friction power (friction.py:31)
+ * correctly rounded and its Manning divisions as IEEE quotients, through shorter routes
  * that lean on the GPU's v_log_f32, v_exp_f32 and v_rcp_f32 where they are exact and on
  * the long sequences elsewhere.  This runs, on the current device, the check that the
- * short routes give the long ones' bits: every float32 of [2^-11, 2^21] for each power;
+ * short routes give the long ones' bits: every float32 of [2^-11, 2^21] for the power;
  * reciprocal scaling over 2^+-100; the division on every significand of the divisor
  * with `a_count` dividend significands (a_count divides 2^23; 2^23 covers every pair,
- * about 35 s); random operand pairs.  counts[12] receives: power mismatches for
- * float32(7/3), (4/3), (1/3); inputs those powers left to the long route; reciprocal
- * scaling mismatches; division significand-pair mismatches; random-pair mismatches;
- * mismatches of the friction helpers against the plain IEEE expressions; the number
- * of power inputs and of significand pairs checked.  All mismatch counts are 0 on a
- * conforming device. */
+ * about 35 s); random division pairs; the whole face update against its plain IEEE
+ * expression on random operands.  counts[8] receives: power mismatches; power inputs
+ * left to the long route; reciprocal scaling mismatches; division significand-pair
+ * mismatches; random-pair mismatches; face-update mismatches; the number of power
+ * inputs and of significand pairs checked.  All mismatch counts are 0 on a conforming
+ * device. */
 int rbs_check_arithmetic(int64_t a_count, uint64_t *counts);
 
 #ifdef __cplusplus

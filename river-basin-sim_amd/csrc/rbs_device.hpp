@@ -107,24 +107,36 @@ __device__ __forceinline__ float pow_third(float x) {
   if (SUB && tiny) p *= (P == 7 ? 0x1p-112 : P == 4 ? 0x1p-64 : 0x1p-16);
   return (float)p;
 }
-// The same correctly rounded power, by a shorter route that is exact where it claims to
-// be and hands every other input to pow_third.  For x in [2^-11, 2^20]:
-//  1. L ~ log2 x (v_log_f32) and y ~ x^(-1/3) = 2^(-L/3) (v_exp_f32), good to about
-//     2^-21; r = 1 - x y^3 exactly as in pow_third, so the series of (1 - r)^(-2/3)
+// wp.pow(h, 7.0/3.0) for a depth h >= H_DRY (friction.py:31)
+__device__ __forceinline__ float pow_e73(float x) { return pow_third<7, false>(x); }
+// wp.pow(R, 4.0/3.0) and wp.pow(R, 1.0/3.0) of a hydraulic radius (friction.py:57,
+// sediment.py:168), which a narrow channel can make arbitrarily small
+__device__ __forceinline__ float pow_e43(float x) { return pow_third<4, true>(x); }
+__device__ __forceinline__ float pow_e13(float x) { return pow_third<1, true>(x); }
+
+// ---- the plain LI face's short routes ------------------------------------------------
+// The local-inertial face evaluates the friction power and two divisions per face.  It
+// reaches the same correctly rounded results by shorter routes built on v_log_f32,
+// v_exp_f32 and v_rcp_f32, each exact where it claims to be and handing every other
+// operand to the long sequences; rbs_check_arithmetic proves them on the device.  (The
+// channel, HLLC and bedload forms keep the long sequences: there the routes' branches
+// cost the scheduler more than their instructions save.)
+//
+// h^float32(7/3) for h in [2^-11, 2^20]:
+//  1. L ~ log2 h (v_log_f32) and y ~ h^(-1/3) = 2^(-L/3) (v_exp_f32), good to about
+//     2^-21; r = 1 - h y^3 exactly as in pow_third, so the series of (1 - r)^(-2/3)
 //     needs r and r^2 only (the r^3 term is below 2^-58);
-//  2. u = d ln x = (d ln 2) L in float32, good to 2.5e-13 absolute (|u| < 1.1e-6);
-//  3. x^q y^2 (1 + g) in float64 as in pow_third, now good to 2^-40.9 relative: within
+//  2. u = d ln h = (d ln 2) L in float32, good to 2.5e-13 absolute (|u| < 1.1e-6);
+//  3. h^3 y^2 (1 + g) in float64 as in pow_third, now good to 2^-40.9 relative: within
 //     2^11.1 float64 ulps of the exact power.
 // Its float32 rounding is therefore the exact power's unless the float64 value lies
 // within 2^13 float64 ulps of a float32 rounding midpoint (the low 29 significand bits
-// within 2^13 of 2^28): such inputs (about 3 in 10^5), and every x outside the range
-// ([2^-10, 2^20] for 4/3 and 1/3, whose radii can be tiny; above 2^20 for 7/3, whose
-// depths are at least H_DRY), take pow_third.  Checked on the device for every float32
-// of [2^-11, 2^21] and each exponent (rbs_check_arithmetic): no input where the short
-// route claims exactness and differs from pow_third.
-template <int P>
-__device__ __forceinline__ float pow_third_hw(float x, bool &exact) {
-  constexpr double d = ((double)P - 3.0 * (double)(float)((double)P / 3.0)) / 3.0;
+// within 2^13 of 2^28): such inputs (about 3 in 10^5) and every h above 2^20 take
+// pow_third (the callers' depths are at least H_DRY > 2^-10).  Checked on the device for
+// every float32 of [2^-11, 2^21]: no input where the short route claims exactness and
+// differs from pow_third.
+__device__ __forceinline__ float pow_e73_hw(float x, bool &exact) {
+  constexpr double d = (7.0 - 3.0 * (double)(float)(7.0 / 3.0)) / 3.0;
   constexpr float ufac = (float)(d * 0.6931471805599453);
   constexpr uint32_t kMargin = 1u << 13;
   const float L = __builtin_amdgcn_logf(x);
@@ -132,36 +144,23 @@ __device__ __forceinline__ float pow_third_hw(float x, bool &exact) {
   const float uf = L * ufac;
   const double a = (double)x, b = (double)yb;
   const double b2 = b * b;
-  const double r = fma(-(a * b), b2, 1.0);
+  const double ab = a * b;  // exact, as is b2
+  const double r = fma(-ab, b2, 1.0);
   const double s2 = r * fma(r, 5.0 / 9.0, 2.0 / 3.0);
   const double u = (double)uf;
   const double g = fma(-u, fma(-0.5, u, s2) + 1.0, s2);
-  // x^q y^2 from the exact x y: (x y)^2 x, (x y)^2, x y^2
-  const double ab = a * b;
-  double A;
-  if (P == 7) A = (ab * ab) * a;
-  else if (P == 4) A = ab * ab;
-  else A = a * b2;
+  const double A = (ab * ab) * a;  // x^3 y^2
   const double p = fma(A, g, A);
   const uint32_t lo = (uint32_t)__double_as_longlong(p) & 0x1fffffffu;
-  // (the friction power's callers pass depths of at least H_DRY > 2^-10)
-  exact = (lo - (0x10000000u - kMargin)) > 2u * kMargin && x <= 0x1p20f &&
-          (P == 7 || x >= 0x1p-10f);
+  exact = (lo - (0x10000000u - kMargin)) > 2u * kMargin && x <= 0x1p20f;
   return (float)p;
 }
-template <int P, bool SUB>
-__device__ __forceinline__ float pow_third_fast(float x) {
+__device__ __forceinline__ float pow_e73_fast(float x) {
   bool exact;
-  float p = pow_third_hw<P>(x, exact);
-  if (!exact) p = pow_third<P, SUB>(x);  // a divergent branch, skipped by most waves
+  float p = pow_e73_hw(x, exact);
+  if (!exact) p = pow_third<7, false>(x);  // a divergent branch, skipped by most waves
   return p;
 }
-// wp.pow(h, 7.0/3.0) for a depth h >= H_DRY (friction.py:31)
-__device__ __forceinline__ float pow_e73(float x) { return pow_third_fast<7, false>(x); }
-// wp.pow(R, 4.0/3.0) and wp.pow(R, 1.0/3.0) of a hydraulic radius (friction.py:57,
-// sediment.py:168), which a narrow channel can make arbitrarily small
-__device__ __forceinline__ float pow_e43(float x) { return pow_third_fast<4, true>(x); }
-__device__ __forceinline__ float pow_e13(float x) { return pow_third_fast<1, true>(x); }
 
 // a / b correctly rounded, by the reciprocal, one Newton step and two Markstein
 // corrections (every remainder exact in an FMA): v_rcp_f32 scales exactly by powers of
@@ -169,8 +168,8 @@ __device__ __forceinline__ float pow_e13(float x) { return pow_third_fast<1, tru
 // the normal range -- |a| in [2^-100, 2^100], |b| in [2^-100, 2^100], a/b normal with
 // room for the remainders.  Checked on the device against the IEEE division on every
 // pair of float32 significands (7.0e13 pairs), and on 2.1e9 random pairs of the ranges
-// the callers use.  Callers check their operands' ranges; the compiler's division (two
-// v_div_scale, v_div_fmas, v_div_fixup around the same steps) stays for the rest.
+// the face uses.  The compiler's division (two v_div_scale, v_div_fmas, v_div_fixup
+// around the same steps) stays for operands outside them.
 __device__ __forceinline__ float div_rn(float a, float b) {
   float y = __builtin_amdgcn_rcpf(b);
   const float e = __builtin_fmaf(-b, y, 1.0f);
@@ -194,60 +193,29 @@ __device__ __forceinline__ float opaque(float v) {
   return v;
 }
 
-// 1 + c / p, each operation rounded as float32, for c >= 0 (a friction numerator) and a
-// positive power p.  c / p by div_rn for c in [2^-50, 2^45] and p in [2^-24, 2^47]
-// (quotients in [2^-97, 2^69]); for c below 2^-50 and p >= 2^-24 the quotient is below
-// 2^-26, and 1 + c / p rounds to 1 exactly as 1 + 0 does; the IEEE division otherwise.
-__device__ __forceinline__ float one_plus_ratio(float c, float p) {
-  const bool p_ok = p >= 0x1p-24f && p <= 0x1p47f;
-  const bool fast = (c <= 0x1p45f && p_ok) || (c < 0x1p-50f && p >= 0x1p-24f);
-  const float xd = opaque(div_rn(c, p));
-  float x = c < 0x1p-50f ? 0.0f : xd;
-  if (!wave_all(fast)) x = fast ? x : c / p;
-  return 1.0f + x;
-}
-// The Manning form for a depth h >= H_DRY: its power p = h^float32(7/3) is then at
-// least 2^-23.3, and at most 2^47 while h <= 2^20.
-__device__ __forceinline__ float one_plus_ratio_h(float c, float p, float h) {
-  const bool fast = c <= 0x1p45f && h <= 0x1p20f;
-  const float xd = opaque(div_rn(c, p));
-  float x = c < 0x1p-50f ? 0.0f : xd;
-  if (!wave_all(fast)) x = fast ? x : c / p;
-  return 1.0f + x;
-}
-
-// a / den for a Manning denominator den = 1 + x, x >= 0 (or NaN): div_rn for |a| in
-// [2^-96, 2^96] and den up to 2^24 (quotients in [2^-120, 2^96]); a zero keeps its sign
-// (den > 0), as the IEEE quotient does; the IEEE division otherwise.
-__device__ __forceinline__ bool den_fast(float a, float den) {
-  const float m = fabsf(a);
-  return den <= 0x1p24f && ((m >= 0x1p-96f && m <= 0x1p96f) || m == 0.0f);
-}
-__device__ __forceinline__ float div_by_den_short(float a, float den) {
-  const float q = opaque(div_rn(a, den));
-  return fabsf(a) == 0.0f ? a : q;
-}
-__device__ __forceinline__ float div_by_den(float a, float den) {
-  const bool fast = den_fast(a, den);
-  float q = div_by_den_short(a, den);
-  if (!wave_all(fast)) q = fast ? q : a / den;
-  return q;
-}
-
 // The Manning face update num / (1 + c / h^float32(7/3)) (local_inertial.py:72-129 with
-// friction.py:18-31) for a depth h >= H_DRY, every operation rounded as float32 in the
-// reference's order: both quotients by the short route, one vote for the wave, and the
-// IEEE divisions for the lanes outside the short route's ranges.
+// friction.py:18-31: c = g dt n^2 |q|) for a depth h >= H_DRY, every operation rounded as
+// float32 in the reference's order, with one vote for the wave:
+//  - p = h^float32(7/3) by pow_e73_fast, in [2^-23.3, 2^47] while h <= 2^20;
+//  - c / p by div_rn for c in [2^-50, 2^45] (quotients in [2^-97, 2^69]); for c below
+//    2^-50 the quotient is below 2^-26 and 1 + c / p rounds to 1, as 1 + 0 does;
+//  - num / den by div_rn for |num| in [2^-96, 2^96] and den = 1 + c / p up to 2^24
+//    (quotients in [2^-120, 2^96]); a zero num is the quotient itself, with its sign
+//    (den > 0), as the IEEE division gives it;
+//  - lanes outside those ranges (and NaN) take the IEEE divisions.
 __device__ __forceinline__ float manning_update(float num, float c, float h) {
-  const float p = pow_e73(h);
+  const float p = pow_e73_fast(h);
   const float xd = opaque(div_rn(c, p));
-  const float x = c < 0x1p-50f ? 0.0f : xd;
-  const float den = 1.0f + x;
-  float q = div_by_den_short(num, den);
-  const bool fast = c <= 0x1p45f && h <= 0x1p20f && den_fast(num, den);
+  const float den = 1.0f + (c < 0x1p-50f ? 0.0f : xd);
+  const float qd = opaque(div_rn(num, den));
+  const float m = fabsf(num);
+  float q = m == 0.0f ? num : qd;
+  const bool fast = c <= 0x1p45f && h <= 0x1p20f && den <= 0x1p24f &&
+                    ((m >= 0x1p-96f && m <= 0x1p96f) || m == 0.0f);
   if (!wave_all(fast)) q = fast ? q : num / (1.0f + c / p);
   return q;
 }
+
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   const double a = (double)x;
   return (float)(a * sqrt(a));
@@ -255,9 +223,8 @@ __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
 
 // friction.py:18-31 manning_denominator, 1 + g dt n^2 |q| / h^e, e = float32(7/3), in
 // the reference's float32 order: the product left to right, one division, one sum.
-// h_flow >= H_DRY (every caller's wet test).
 __device__ __forceinline__ float manning_den(float q_prev, float h_flow, float n, float g, float dt) {
-  return one_plus_ratio_h(g * dt * n * n * fabsf(q_prev), pow_e73(h_flow), h_flow);
+  return 1.0f + g * dt * n * n * fabsf(q_prev) / pow_e73(h_flow);
 }
 
 // sources.py:66-82 kahan_add
@@ -357,7 +324,7 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
   const float h_fp = eta_max - z_bank;
   if (!(h_fp < kHDry)) {
     const float num = q_fp - g * h_fp * dt * slope;
-    fp = div_by_den(num, manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt));
+    fp = num / manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt);
   }
   const float w_face = fmn(a.w, b.w);
   float ch = 0.0f;
@@ -368,8 +335,8 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
       const float radius = w_face * h_ch / (w_face + 2.0f * h_ch);
       const float nc = 0.5f * (a.nc + b.nc);
       const float num = q_ch - g * h_ch * dt * slope;
-      const float den = one_plus_ratio(g * dt * nc * nc * fabsf(q_ch), h_ch * pow_e43(radius));
-      ch = div_by_den(num, den);
+      const float den = 1.0f + g * dt * nc * nc * fabsf(q_ch) / (h_ch * pow_e43(radius));
+      ch = num / den;
     }
   }
   q_fp = fp;

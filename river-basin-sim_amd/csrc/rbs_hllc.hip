@@ -431,8 +431,8 @@ __global__ __launch_bounds__(kThreads) void hllc_stage_kernel(HlArgs a) {
         } else {
           const float mom = sqrtf(hu * hu + hv * hv);
           const float den = manning_den(mom, hh, a.n[o], g, dt);
-          hu = div_by_den(hu, den);
-          hv = div_by_den(hv, den);
+          hu = hu / den;
+          hv = hv / den;
         }
         // sources (:1350-1373)
         const float rain = ctl->rain;
@@ -885,8 +885,8 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
           } else {
             const float mom = sqrtf(hu * hu + hv * hv);
             const float den = manning_den(mom, hh, a.n[o], g, dt);
-            hu = div_by_den(hu, den);
-            hv = div_by_den(hv, den);
+            hu = hu / den;
+            hv = hv / den;
           }
           // sources (:1350-1373)
           const float rain = ctl->rain;

@@ -626,21 +626,29 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
     // prefetched row: h, z, n, old qx, old qy; and the next finished row's sources
     float ph, pz, pn, pqx, pqy;
     float pcomp = 0.0f, prain = 0.0f, pinfil = 0.0f;
+    // The lane offsets are made opaque at every use (an empty asm), as for the row
+    // blocks' stores: otherwise the compiler hoists one 64-bit per-lane pointer per
+    // field out of the loop (16 registers for the eight fields read), spills one, and
+    // the reload's wait for every load in flight stalls the next row's prefetch.
     auto ingest = [&](int R) {
       const int64_t o = (int64_t)clampi(R, 0, ny - 1) * P;  // uniform
       const int64_t oy = (int64_t)clampi(R, 0, ny) * P;
-      ph = ROWS ? ld_nt(at(h_in + o, cc)) : at(h_in + o, cc);
-      pz = at(zf + o, cc);
-      pn = at(nf + o, cc);
-      pqx = ROWS ? ld_nt(at(qx_in + o, cf)) : at(qx_in + o, cf);
-      pqy = ROWS ? ld_nt(at(qy_in + oy, cc)) : at(qy_in + oy, cc);
+      uint32_t kc = cc, kf = cf;
+      asm volatile("" : "+v"(kc), "+v"(kf));
+      ph = ROWS ? ld_nt(at(h_in + o, kc)) : at(h_in + o, kc);
+      pz = at(zf + o, kc);
+      pn = at(nf + o, kc);
+      pqx = ROWS ? ld_nt(at(qx_in + o, kf)) : at(qx_in + o, kf);
+      pqy = ROWS ? ld_nt(at(qy_in + oy, kc)) : at(qy_in + oy, kc);
     };
     auto sources = [&](int i) {
       if (i >= y0 && i < y_end && own) {
         const int64_t o = (int64_t)i * P;  // uniform
-        if (flags & LI_COMP) pcomp = ROWS ? ld_nt(at(hcomp + o, cu)) : at(hcomp + o, cu);
-        if (flags & LI_FIELD) prain = at(rainf + o, cu);
-        if (flags & LI_INFIL) pinfil = at(infilf + o, cu);
+        uint32_t ku = cu;
+        asm volatile("" : "+v"(ku));
+        if (flags & LI_COMP) pcomp = ROWS ? ld_nt(at(hcomp + o, ku)) : at(hcomp + o, ku);
+        if (flags & LI_FIELD) prain = at(rainf + o, ku);
+        if (flags & LI_INFIL) pinfil = at(infilf + o, ku);
       }
     };
     // row y0-2 only feeds the y-faces above row y0-1: take its inputs, no faces
