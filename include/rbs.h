@@ -362,22 +362,6 @@ int rbs_batch_get_variant(rbs_batch *b, int32_t *variant, int32_t *variant_sed);
 int rbs_batch_stats(rbs_batch *b, int64_t *rebuilds, double *rebuild_ms,
                     int64_t *graph_launches, double *gpu_ms);
 
-/* ---- self-check of the device arithmetic (no reference counterpart) ------------------ */
-/* The local-inertial face evaluates the reference's friction power (friction.py:31)
- * correctly rounded and its Manning divisions as IEEE quotients, through shorter routes
- * that lean on the GPU's v_log_f32, v_exp_f32 and v_rcp_f32 where they are exact and on
- * the long sequences elsewhere.  This runs, on the current device, the check that the
- * short routes give the long ones' bits: every float32 of [2^-11, 2^21] for the power;
- * reciprocal scaling over 2^+-100; the division on every significand of the divisor
- * with `a_count` dividend significands (a_count divides 2^23; 2^23 covers every pair,
- * about 35 s); random division pairs; the whole face update against its plain IEEE
- * expression on random operands.  counts[8] receives: power mismatches; power inputs
- * left to the long route; reciprocal scaling mismatches; division significand-pair
- * mismatches; random-pair mismatches; face-update mismatches; the number of power
- * inputs and of significand pairs checked.  All mismatch counts are 0 on a conforming
- * device. */
-int rbs_check_arithmetic(int64_t a_count, uint64_t *counts);
-
 #ifdef __cplusplus
 }
 #endif

@@ -114,107 +114,8 @@ __device__ __forceinline__ float pow_e73(float x) { return pow_third<7, false>(x
 __device__ __forceinline__ float pow_e43(float x) { return pow_third<4, true>(x); }
 __device__ __forceinline__ float pow_e13(float x) { return pow_third<1, true>(x); }
 
-// ---- the plain LI face's short routes ------------------------------------------------
-// The local-inertial face evaluates the friction power and two divisions per face.  It
-// reaches the same correctly rounded results by shorter routes built on v_log_f32,
-// v_exp_f32 and v_rcp_f32, each exact where it claims to be and handing every other
-// operand to the long sequences; rbs_check_arithmetic proves them on the device.  (The
-// channel, HLLC and bedload forms keep the long sequences: there the routes' branches
-// cost the scheduler more than their instructions save.)
-//
-// h^float32(7/3) for h in [2^-11, 2^20]:
-//  1. L ~ log2 h (v_log_f32) and y ~ h^(-1/3) = 2^(-L/3) (v_exp_f32), good to about
-//     2^-21; r = 1 - h y^3 exactly as in pow_third, so the series of (1 - r)^(-2/3)
-//     needs r and r^2 only (the r^3 term is below 2^-58);
-//  2. u = d ln h = (d ln 2) L in float32, good to 2.5e-13 absolute (|u| < 1.1e-6);
-//  3. h^3 y^2 (1 + g) in float64 as in pow_third, now good to 2^-40.9 relative: within
-//     2^11.1 float64 ulps of the exact power.
-// Its float32 rounding is therefore the exact power's unless the float64 value lies
-// within 2^13 float64 ulps of a float32 rounding midpoint (the low 29 significand bits
-// within 2^13 of 2^28): such inputs (about 3 in 10^5) and every h above 2^20 take
-// pow_third (the callers' depths are at least H_DRY > 2^-10).  Checked on the device for
-// every float32 of [2^-11, 2^21]: no input where the short route claims exactness and
-// differs from pow_third.
-__device__ __forceinline__ float pow_e73_hw(float x, bool &exact) {
-  constexpr double d = (7.0 - 3.0 * (double)(float)(7.0 / 3.0)) / 3.0;
-  constexpr float ufac = (float)(d * 0.6931471805599453);
-  constexpr uint32_t kMargin = 1u << 13;
-  const float L = __builtin_amdgcn_logf(x);
-  const float yb = __builtin_amdgcn_exp2f(L * (-1.0f / 3.0f));
-  const float uf = L * ufac;
-  const double a = (double)x, b = (double)yb;
-  const double b2 = b * b;
-  const double ab = a * b;  // exact, as is b2
-  const double r = fma(-ab, b2, 1.0);
-  const double s2 = r * fma(r, 5.0 / 9.0, 2.0 / 3.0);
-  const double u = (double)uf;
-  const double g = fma(-u, fma(-0.5, u, s2) + 1.0, s2);
-  const double A = (ab * ab) * a;  // x^3 y^2
-  const double p = fma(A, g, A);
-  const uint32_t lo = (uint32_t)__double_as_longlong(p) & 0x1fffffffu;
-  exact = (lo - (0x10000000u - kMargin)) > 2u * kMargin && x <= 0x1p20f;
-  return (float)p;
-}
-__device__ __forceinline__ float pow_e73_fast(float x) {
-  bool exact;
-  float p = pow_e73_hw(x, exact);
-  if (!exact) p = pow_third<7, false>(x);  // a divergent branch, skipped by most waves
-  return p;
-}
-
-// a / b correctly rounded, by the reciprocal, one Newton step and two Markstein
-// corrections (every remainder exact in an FMA): v_rcp_f32 scales exactly by powers of
-// two over 2^-100..2^100, so the result is the IEEE quotient whenever no step leaves
-// the normal range -- |a| in [2^-100, 2^100], |b| in [2^-100, 2^100], a/b normal with
-// room for the remainders.  Checked on the device against the IEEE division on every
-// pair of float32 significands (7.0e13 pairs), and on 2.1e9 random pairs of the ranges
-// the face uses.  The compiler's division (two v_div_scale, v_div_fmas, v_div_fixup
-// around the same steps) stays for operands outside them.
-__device__ __forceinline__ float div_rn(float a, float b) {
-  float y = __builtin_amdgcn_rcpf(b);
-  const float e = __builtin_fmaf(-b, y, 1.0f);
-  y = __builtin_fmaf(e, y, y);
-  float q = a * y;
-  float r = __builtin_fmaf(-b, q, a);
-  q = __builtin_fmaf(r, y, q);
-  r = __builtin_fmaf(-b, q, a);
-  return __builtin_fmaf(r, y, q);
-}
-
-// The rare operands outside div_rn's ranges take the IEEE division in a branch on the
-// whole wave's vote, which the compiler keeps as a skipped scalar branch (a per-lane
-// condition it would evaluate speculatively on every lane).  The vote is one ballot of
-// the lanes that need the long route.
+// A wave-uniform vote: one ballot of the lanes for which `p` fails.
 __device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
-// Pins a value as computed on every lane: the short quotients are cheaper than the
-// branch the compiler would otherwise wrap around the one arm of a select that uses them.
-__device__ __forceinline__ float opaque(float v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
-// The Manning face update num / (1 + c / h^float32(7/3)) (local_inertial.py:72-129 with
-// friction.py:18-31: c = g dt n^2 |q|) for a depth h >= H_DRY, every operation rounded as
-// float32 in the reference's order, with one vote for the wave:
-//  - p = h^float32(7/3) by pow_e73_fast, in [2^-23.3, 2^47] while h <= 2^20;
-//  - c / p by div_rn for c in [2^-50, 2^45] (quotients in [2^-97, 2^69]); for c below
-//    2^-50 the quotient is below 2^-26 and 1 + c / p rounds to 1, as 1 + 0 does;
-//  - num / den by div_rn for |num| in [2^-96, 2^96] and den = 1 + c / p up to 2^24
-//    (quotients in [2^-120, 2^96]); a zero num is the quotient itself, with its sign
-//    (den > 0), as the IEEE division gives it;
-//  - lanes outside those ranges (and NaN) take the IEEE divisions.
-__device__ __forceinline__ float manning_update(float num, float c, float h) {
-  const float p = pow_e73_fast(h);
-  const float xd = opaque(div_rn(c, p));
-  const float den = 1.0f + (c < 0x1p-50f ? 0.0f : xd);
-  const float qd = opaque(div_rn(num, den));
-  const float m = fabsf(num);
-  float q = m == 0.0f ? num : qd;
-  const bool fast = c <= 0x1p45f && h <= 0x1p20f && den <= 0x1p24f &&
-                    ((m >= 0x1p-96f && m <= 0x1p96f) || m == 0.0f);
-  if (!wave_all(fast)) q = fast ? q : num / (1.0f + c / p);
-  return q;
-}
 
 __device__ __forceinline__ float pow15(float x) {  // x^(3/2) = x * sqrt(x)
   const double a = (double)x;
@@ -269,7 +170,8 @@ __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, floa
   float n_face = 0.5f * (n_a + n_b);
   float slope = div_uniform(eta_b - eta_a, dx, rdx);
   float num = q - g * h_flow * dt * slope;
-  return manning_update(num, g * dt * n_face * n_face * fabsf(q), h_flow);  // manning_den
+  float den = manning_den(q, h_flow, n_face, g, dt);
+  return num / den;
 }
 
 // ---- sub-grid channels (channels.py:59-138, local_inertial.py:266-500) -------------

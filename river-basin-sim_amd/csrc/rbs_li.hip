@@ -1185,6 +1185,8 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
     c1 = ChanCell{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};
     h1 = ph;
     ingest(y0 - 1);
+    // four rows per pass, as li_strip (348 -> 343 us at 4096^2)
+    _Pragma("unroll 4")
     for (int it = 1; it < H + 4; ++it) {
       const int R = y0 - 2 + it;
       const ChanCell c0{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};

@@ -157,7 +157,6 @@ SIGNATURES: dict[str, tuple] = {
     "rbs_batch_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), _pdbl,
                                        ctypes.POINTER(ctypes.c_int64), _pdbl]),
     "rbs_time_li_kernel": (ctypes.c_int, [_vp, _i32, _dbl, _dbl, _dbl, _pdbl]),
-    "rbs_check_arithmetic": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib: ctypes.CDLL | None = None
