@@ -36,6 +36,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "Gcell-updates/s + %HBM peak; ensemble scenarios/h & p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (datasheet)
 BYTES_PER_CELL = 48  # h r+w, qx r+w, qy r+w, z, n, rain, infil, h_comp r+w
 
 
@@ -1040,6 +1041,7 @@ def bench_hllc(args) -> int:
     sweep = hllc_sweep(st, dt_k, n * n) if args.sweep else None
     st.close()
     wet_sweep = hllc_wet_sweep(bed, dx) if args.sweep else None
+    wet = hllc_wet(bed, dx)
     early = {"steps": taken, "wall_s": round(el, 3), "gcell_s": round(value, 3),
              "t_end_s": round(led.t, 1), "stage_kernels_us_per_step": round(step_us, 1)}
     flood = hllc_flood(args, bed, depth, dx, stage, cells, hg) if args.hllc_hours > 0 else None
@@ -1065,6 +1067,7 @@ def bench_hllc(args) -> int:
                    "variant": args.hllc_variant if args.hllc_variant >= 0 else "default"},
         "roofline": hllc_roofline(n, args.hllc_variant, step_us, bytes_cell),
         "lake_at_rest": lake,
+        "wet_lake": wet,
         "early_window": early,
         "flood": flood,
     }
@@ -1092,21 +1095,62 @@ def hllc_roofline(n: int, variant: int, step_us: float, bytes_cell: int) -> dict
     return out
 
 
-def hllc_wet_sweep(bed, dx: float) -> list:
-    """Every HLLC variant on the valley's bed under a lake 1 m above its highest point:
-    every cell wet, every face computed (the dry paths never apply)."""
-    from rbs import _lib
+def wet_lake(bed, dx: float):
+    """The valley's bed under a lake 1 m above its highest point: every cell wet, every
+    face computed (the dry paths never apply)."""
     from rbs.core import hllc
     from rbs.core.state import State
 
     eta = float(bed.max()) + 1.0
     st = State.from_bed(bed, dx, depth=(eta - bed.astype(np.float64)).astype(np.float32),
                         manning=0.035)
-    dt = hllc.compute_dt(st, 0.45, 30.0)
+    return st, hllc.compute_dt(st, 0.45, 30.0), eta
+
+
+def hllc_wet_sweep(bed, dx: float) -> list:
+    """Every HLLC variant on the wet lake over the valley's bed."""
+    st, dt, eta = wet_lake(bed, dx)
     log(f"[hllc] wet sweep: lake at {eta:.1f} m over the {bed.shape[0]}^2 valley, dt {dt:.3f} s")
     rows = hllc_sweep(st, dt, bed.size)
     st.close()
     return rows
+
+
+def hllc_wet(bed, dx: float) -> dict:
+    """The library's HLLC kernels on the wet lake, where every face is computed.  Both
+    fractions: HBM at 32 B per cell, and the FP32 vector rate of the stage kernels'
+    FLOPs (SQ_INSTS_VALU_FLOPS_FP32 per launch on this state, x 64 lanes,
+    `profiles/hllc_wet_lake_8192_valu.json`) over the step's time, against the
+    157.3 TFLOP/s FP32 vector peak."""
+    from rbs import _lib
+
+    st, dt, eta = wet_lake(bed, dx)
+    avg = ctypes.c_double()
+    _lib.call("rbs_time_hllc_step", st._handle, 2, dt, ctypes.byref(avg))
+    _lib.call("rbs_time_hllc_step", st._handle, 10, dt, ctypes.byref(avg))
+    st.close()
+    us = avg.value * 1e3
+    cells = bed.size
+    achieved = 32 * cells / (us * 1e-6) / 1e9
+    out = {"grid": f"{bed.shape[0]}x{bed.shape[1]}", "lake_m": round(eta, 1),
+           "step_us": round(us, 1), "gcell_s": round(cells / us / 1e3, 2),
+           "hbm": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell": 32},
+           "valu": None}
+    vf = ROOT / "profiles" / "hllc_wet_lake_8192_valu.json"
+    if vf.exists() and bed.shape == (8192, 8192):
+        v = json.loads(vf.read_text())["kernels"].values()
+        flops = 64 * sum(k["fp32_flops_per_wave_inst"] for k in v)
+        tflops = flops / (us * 1e-6) / 1e12
+        out["valu"] = {"bound": "fp32 vector", "achieved": round(tflops, 2),
+                       "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                       "fp32_flops_per_cell_step": round(flops / cells, 1),
+                       "valu_insts_per_cell_step": round(64 * sum(k["valu_insts"] for k in v) / cells, 1)}
+    log(f"[hllc] wet lake {out['grid']}: {us:.0f} us per step, {out['gcell_s']} Gcell/s, "
+        f"HBM {100 * out['hbm']['frac']:.1f}%"
+        + (f", FP32 VALU {100 * out['valu']['frac']:.1f}%" if out["valu"] else ""))
+    return out
 
 
 def dry_block_fraction(h, rows: int = 16, cols: int = 58, halo: int = 3) -> float:
