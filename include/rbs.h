@@ -350,10 +350,11 @@ int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is
  * the state. */
 int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
 /* The LI kernel variants a batch launches: `variant` for members without the bedload
- * integral, `variant_sed` for members with it (library defaults unless set); takes
- * effect at the next advance.  Members without the bedload integral that are at
- * least 2048 cells wide run the 50-row row blocks whatever `variant` says, as they
- * would alone; the three groups launch side by side in the batch's graph. */
+ * integral, `variant_sed` for members with it (unless set: the 50-row row blocks and
+ * the 8-row strips); takes effect at the next advance.  Members without the bedload
+ * integral that are at least 2048 cells wide run the 50-row row blocks whatever
+ * `variant` says, as they would alone; the three groups launch side by side in the
+ * batch's graph. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed);
 int rbs_batch_get_variant(rbs_batch *b, int32_t *variant, int32_t *variant_sed);
 /* Host-side cost of membership changes so far (table rebuilds and graph captures:

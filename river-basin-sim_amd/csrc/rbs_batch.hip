@@ -24,7 +24,7 @@ struct rbs_batch {
   // joined to `stream` inside the captured graph, beside the plain segment's
   hipStream_t side = nullptr, side_wide = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr, join_wide_ev = nullptr;
-  int variant = li_default_variant();          // members without the bedload integral
+  int variant = li_default_batch_variant();    // members without the bedload integral
   int variant_sed = li_default_sed_variant();  // members with it
   int chunk = 16;
   // wide members without the bedload integral (at least li_wide_batch_min_nx()

@@ -240,6 +240,7 @@ void hllc_variant_shape(int v, int *bx, int *by);
 int li_variant_count();
 int li_default_variant();
 int li_wide_batch_variant();  // the row blocks a batch gives its wide plain members
+int li_default_batch_variant();  // a batch's members without the bedload integral
 int li_wide_batch_min_nx();
 int li_initial_variant(int nx);
 int chan_variant_count();

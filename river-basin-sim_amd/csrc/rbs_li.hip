@@ -1359,6 +1359,12 @@ int li_default_variant() { return kDefaultLiVariant; }
 // default), the library default otherwise.
 int li_initial_variant(int nx) { return nx >= 2048 ? kRowsVariantForWide : kDefaultLiVariant; }
 int li_wide_batch_variant() { return kRowsVariantForWide; }
+// a batch's members without the bedload integral: the row blocks too.  The 256
+// scenarios of sides 259-2027 took 5.9 s batched on them against 6.8 s on 32-row
+// strips (6.1 s with the row blocks from 1024 columns up, 5.95 s from 512): the
+// narrow members idle most of a 480-column block's waves, but their time is small
+// beside the large members', whose halo re-reads the 50-row blocks cut.
+int li_default_batch_variant() { return kRowsVariantForWide; }
 int li_wide_batch_min_nx() { return 2048; }
 // batches of bedload members: strips of 8 rows.  Strips beat the LDS tiles inside a
 // batch, where the members' tiles fill the GPU whatever their shapes; the bedload
