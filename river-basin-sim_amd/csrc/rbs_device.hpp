@@ -160,8 +160,28 @@ __device__ __forceinline__ float div_uniform(float a, float b, float rb) {
   return normal ? q1 : a / b;
 }
 
+// c / p of the LI face's friction term (friction.py:31: g dt n^2 |q| / h^e), for c >= 0
+// and p = h^e with h >= kHDry.  The compiler's IEEE float division is this sequence
+// with v_div_scale around it, and v_div_scale changes nothing unless c < 2^-103, c/p
+// is subnormal or above 2^96, or p > 2^125.  The first two give c/p below 2^-80
+// (p >= 1e-7), where the division's last bits vanish in 1 + c/p = 1; the others need
+// |q| beyond 1e19 m^2/s or a flow depth beyond 1e16 m.  v_div_fixup keeps the special
+// cases (0, inf, NaN) of the IEEE quotient.  Two instructions fewer than the division.
+__device__ __forceinline__ float friction_ratio(float c, float p) {
+  float y = __builtin_amdgcn_rcpf(p);
+  const float e = __builtin_fmaf(-p, y, 1.0f);
+  y = __builtin_fmaf(e, y, y);
+  float q = c * y;
+  float r = __builtin_fmaf(-p, q, c);
+  q = __builtin_fmaf(r, y, q);
+  r = __builtin_fmaf(-p, q, c);
+  q = __builtin_fmaf(r, y, q);
+  return __builtin_amdgcn_div_fixupf(q, p, c);
+}
+
 // local_inertial.py:72-99 (update_qx) / :102-129 (update_qy): one interior face.
 // a = the left/top cell, b = the right/bottom cell; rdx = RN(1/dx).
+// The friction term is manning_den's, left to right, with friction_ratio's quotient.
 __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, float z_a, float z_b,
                                          float n_a, float n_b, float dx, float rdx, float dt) {
   const float g = kG;
@@ -170,7 +190,7 @@ __device__ __forceinline__ float li_face(float q, float eta_a, float eta_b, floa
   float n_face = 0.5f * (n_a + n_b);
   float slope = div_uniform(eta_b - eta_a, dx, rdx);
   float num = q - g * h_flow * dt * slope;
-  float den = manning_den(q, h_flow, n_face, g, dt);
+  float den = 1.0f + friction_ratio(g * dt * n_face * n_face * fabsf(q), pow_e73(h_flow));
   return num / den;
 }
 

@@ -85,16 +85,17 @@ __device__ __forceinline__ float donor_beta(float ql, float qr, float qt, float 
 
 // limit_qx / limit_qy (:242-263): each face scaled by its donor's factor.
 // bc = this cell, bl/br/bt/bb = left/right/top/bottom neighbours.
+// The reference leaves a zero face untouched; here a face that is not positive is
+// scaled by the downstream donor's factor, which is the same bits: donor_beta returns
+// a finite factor in [0, 1], and +-0 times such a factor is +-0 (a NaN face stays
+// NaN).  One compare and one select per face, no branch (at 4096^2 the branches cost
+// 3.5 % of the step kernel).
 __device__ __forceinline__ void limit_faces(float &qxL, float &qxR, float &qyT, float &qyB,
                                             float bc, float bl, float br, float bt, float bb) {
-  if (qxL > 0.0f) qxL = qxL * bl;
-  else if (qxL < 0.0f) qxL = qxL * bc;
-  if (qxR > 0.0f) qxR = qxR * bc;
-  else if (qxR < 0.0f) qxR = qxR * br;
-  if (qyT > 0.0f) qyT = qyT * bt;
-  else if (qyT < 0.0f) qyT = qyT * bc;
-  if (qyB > 0.0f) qyB = qyB * bc;
-  else if (qyB < 0.0f) qyB = qyB * bb;
+  qxL = qxL * (qxL > 0.0f ? bl : bc);
+  qxR = qxR * (qxR > 0.0f ? bc : br);
+  qyT = qyT * (qyT > 0.0f ? bt : bc);
+  qyB = qyB * (qyB > 0.0f ? bc : bb);
 }
 
 struct CellSums {
@@ -463,14 +464,16 @@ __device__ __forceinline__ void st_nt(R &ref, float v) {
 
 // The neighbouring lane's value through a DPP wave-wide shift (one VALU move, no LDS
 // crossbar): lane i takes lane i-1's (wave_shr:1) or lane i+1's (wave_shl:1) value,
-// and the lane with no neighbour keeps its own, as __shfl_up / __shfl_down return.
+// and the lane with no neighbour (lane 0, lane 63) reads 0 (bound_ctrl), so no copy of
+// the old value is needed.  What lanes 0 and 63 compute from the 0 (a face, then a
+// donor factor) reaches lanes 1 and 62 at most, which write nothing either.
 __device__ __forceinline__ float from_left(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                    0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138 /* wave_shr:1 */,
+                                                 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                    0x130 /* wave_shl:1 */, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130 /* wave_shl:1 */,
+                                                 0xF, 0xF, true));
 }
 
 // NW > 0 ("row blocks"): the block's NW waves sit side by side on the same H rows,
@@ -1052,14 +1055,14 @@ static const LiVariant kLiVariants[] = {
     RBS_LI_TILE(64, 16),     RBS_LI_TILE(64, 8),      RBS_LI_TILE(64, 32),
     RBS_LI_TILE(128, 8),     RBS_LI_TILE(128, 16),    RBS_LI_TILE(32, 32),
     RBS_LI_STRIP(16, 6),     RBS_LI_STRIP(32, 6),     RBS_LI_STRIP(64, 6),
-    RBS_LI_ROWS(16, 6, 8, 2), RBS_LI_ROWS(32, 6, 8, 2), RBS_LI_ROWS(50, 6, 8, 2),
+    RBS_LI_ROWS(16, 6, 8, 2), RBS_LI_ROWS(32, 6, 8, 2), RBS_LI_ROWS(49, 6, 8, 2),
     // 8-row strips: a short serial chain per wave for grids where a few wet tiles
     // among skipped dry ones set the step's time
     RBS_LI_STRIP(8, 6),
 };
 constexpr int kNumLiVariants = sizeof(kLiVariants) / sizeof(kLiVariants[0]);
 constexpr int kDefaultLiVariant = 7;    // strips of 32 rows
-constexpr int kRowsVariantForWide = 11;  // row blocks of 50 rows, two rows per barrier
+constexpr int kRowsVariantForWide = 11;  // row blocks of 49 rows, two rows per barrier
 
 // ---- family 4: register strips of a state with sub-grid channels ----------------------
 // li_strip's rolling window with the channel forms: each row's cells carry the
@@ -1353,17 +1356,21 @@ static const LiVariant &variant_at(int v) {
 
 int li_variant_count() { return kNumLiVariants; }
 int li_default_variant() { return kDefaultLiVariant; }
-// A new state's variant: the row blocks of 50 rows on grids at least 2048 cells wide
+// A new state's variant: the row blocks of 49 rows on grids at least 2048 cells wide
 // (where a ragged last block idles few of their waves; at 4096^2 they re-read the
 // fewest halo rows, 205 us against 214 us for 16-row blocks and 222 us for the
-// default), the library default otherwise.
+// default, measured with 50-row blocks), the library default otherwise.  49 rows:
+// at 4096^2 the 9 x 84 = 756 blocks of eight waves all fit on the GPU at once (six
+// waves per SIMD, 768 blocks), three on most CUs, and each of them walks 52 rows
+// instead of 50-row blocks' 53 (1.5 us less per step); 48 rows give 774 blocks, and
+// the six left over run after the rest (199 us).
 int li_initial_variant(int nx) { return nx >= 2048 ? kRowsVariantForWide : kDefaultLiVariant; }
 int li_wide_batch_variant() { return kRowsVariantForWide; }
 // a batch's members without the bedload integral: the row blocks too.  The 256
 // scenarios of sides 259-2027 took 5.9 s batched on them against 6.8 s on 32-row
 // strips (6.1 s with the row blocks from 1024 columns up, 5.95 s from 512): the
 // narrow members idle most of a 480-column block's waves, but their time is small
-// beside the large members', whose halo re-reads the 50-row blocks cut.
+// beside the large members', whose halo re-reads the tall row blocks cut.
 int li_default_batch_variant() { return kRowsVariantForWide; }
 int li_wide_batch_min_nx() { return 2048; }
 // batches of bedload members: strips of 8 rows.  Strips beat the LDS tiles inside a
