@@ -54,8 +54,9 @@ static inline float fclamp(float x, float lo, float hi) { return fmn(fmx(x, lo),
  * p/3 - d, by a fixed sequence of IEEE basic operations (fused multiply-adds only where
  * written as fmaf/fma, which are correctly rounded), so it is the same bits on every
  * IEEE machine; the HIP kernels evaluate the same sequence:
- *   y ~ x^(-1/3): a bit-pattern seed, two float32 Newton steps (relative error < 2e-5);
- *     r = 1 - x y^3 exactly in float64;
+ *   y ~ x^(-1/3): a bit-pattern seed y0 times the series of (1 - r0)^(-1/3) to r0^3,
+ *     r0 = 1 - x y0^3, in float32 (|1 - x y^3| < 8.6e-5); r = 1 - x y^3 exactly in
+ *     float64;
  *   x^e = x^q y^2 (1 - r)^(-2/3) x^(-d), q = 3, 2, 1 for p = 7, 4, 1: the binomial
  *     series of (1 - r)^(-2/3) to r^3 and exp(-u) = 1 - u + u^2/2, u = d ln x, merged
  *     into one factor 1 + g; ln x from a float32 polynomial (3e-8 absolute, so u is good
@@ -71,12 +72,9 @@ static inline float cbrt_seed(float x) {
   uint32_t j = 0x54a2fa8cu - i / 3u;
   float y;
   memcpy(&y, &j, 4);
-  const float third = (float)(1.0 / 3.0);
-  for (int k = 0; k < 2; ++k) {
-    float r = fmaf(-(x * y), y * y, 1.0f);
-    y = fmaf(y * r, third, y);
-  }
-  return y;
+  float r = fmaf(-(x * y), y * y, 1.0f);
+  float t = fmaf(fmaf(r, (float)(14.0 / 81.0), (float)(2.0 / 9.0)), r, (float)(1.0 / 3.0));
+  return fmaf(y * r, t, y);
 }
 /* ln x, x = 2^k m normal with m in [sqrt(1/2), sqrt(2)), k shifted by kadj: the bit
  * pattern offset by that of sqrt(1/2) splits off k and m; ln(1 + t) = t - t^2/2 +

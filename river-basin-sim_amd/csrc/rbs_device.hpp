@@ -35,25 +35,25 @@ __device__ __forceinline__ gptr<T> glob(T *p) {
 // by one fixed sequence of IEEE basic operations (the build sets -ffp-contract=off, so
 // the only fused multiply-adds are the fmaf/fma written here, correctly rounded
 // everywhere), bit-identical on every IEEE machine:
-//  1. y ~ x^(-1/3) from a bit-pattern seed and two float32 Newton steps (relative
-//     error below 2e-5); then r = 1 - x y^3 exactly in float64 (x y and y^2 are exact);
+//  1. y ~ x^(-1/3) from a bit-pattern seed and one float32 fourth-order step (|1 - x
+//     y^3| below 8.6e-5 for every normal x); then r = 1 - x y^3 exactly in float64 (x
+//     y and y^2 are exact);
 //  2. x^(p/3 - d) = x^q y^2 (1 - r)^(-2/3) x^(-d), since x^(-2/3) = y^2 (1 - r)^(-2/3),
 //     with q = 3, 2, 1 for p = 7, 4, 1: the binomial series of (1 - r)^(-2/3) to r^3 (the next term
-//     is below 6e-18) and x^(-d) = exp(-u) = 1 - u + u^2/2, u = d ln x (|u| < 1e-5 over
+//     is below 2.5e-17) and x^(-d) = exp(-u) = 1 - u + u^2/2, u = d ln x (|u| < 1e-5 over
 //     the float32 range), merged into one factor 1 + g, with ln x to 3e-8 absolute
 //     (below), so u is good to 2.4e-15 relative;
 //  3. x^q y^2 (1 + g) in float64, rounded once to float32.
 // Float64 here buys the last float32 bit of a power; every field stays float32.
-__device__ __forceinline__ float cbrt_seed(float x) {  // x^(-1/3) to 2e-5
+// The seed y0 (|1 - x y0^3| < 0.13) times the series of (1 - r0)^(-1/3) = 1 + r0/3 +
+// 2 r0^2/9 + 14 r0^3/81, r0 = 1 - x y0^3: seven float32 operations (two Newton steps
+// take ten and leave |r| below 5.6e-5, which moves no power in [1e-3, 1e4)).
+__device__ __forceinline__ float cbrt_seed(float x) {  // x^(-1/3) to 2.9e-5
   const uint32_t j = 0x54a2fa8cu - __float_as_uint(x) / 3u;
-  float y = __uint_as_float(j);
-  const float third = (float)(1.0 / 3.0);
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const float r = fmaf(-(x * y), y * y, 1.0f);
-    y = fmaf(y * r, third, y);
-  }
-  return y;
+  const float y = __uint_as_float(j);
+  const float r = fmaf(-(x * y), y * y, 1.0f);
+  const float t = fmaf(fmaf(r, (float)(14.0 / 81.0), (float)(2.0 / 9.0)), r, (float)(1.0 / 3.0));
+  return fmaf(y * r, t, y);
 }
 // ln x for a positive normal float32 x = 2^k m, m in [sqrt(1/2), sqrt(2)) (k shifted by
 // kadj), to 3e-8 absolute: the bit pattern offset by that of sqrt(1/2) gives k and m;
