@@ -314,12 +314,13 @@ int rbs_batch_member_active(rbs_batch *b, int32_t slot, int32_t *active);
 
 /* ---- measurement and tuning (no reference counterpart) ------------------------------------- */
 /* Run `reps` host-driven LI steps of `dt` (as rbs_li_step with limit on), timing each
- * fused step kernel with HIP events recorded on the state's own stream; *avg_ms
- * receives the mean kernel duration.  Advances the state. */
+ * fused step kernel with HIP events on the state's own stream, stamped by the kernel's
+ * dispatch (hipExtLaunchKernelGGL's start and stop events); *avg_ms receives the mean
+ * kernel duration.  Advances the state. */
 int rbs_time_li_kernel(rbs_state *st, int32_t reps, double dt, double rain, double rain_scale,
                        double *avg_ms);
 /* Select the LI kernel variant, 0 <= variant < rbs_li_variant_count() (a new state
- * starts on the library's default, or on the 50-row row blocks when it is at least
+ * starts on the library's default, or on the 49-row row blocks when it is at least
  * 2048 cells wide).  Returns RBS_E_ARG for an unknown variant.
  * Depths and faces are bitwise identical across variants (the ledger's volume sum
  * may differ in its last bits: its partials follow the tile shape). */
@@ -350,9 +351,9 @@ int rbs_hllc_variant_info(int32_t variant, int32_t *bx, int32_t *by, int32_t *is
  * the state. */
 int rbs_time_hllc_step(rbs_state *st, int32_t reps, double dt, double *avg_ms);
 /* The LI kernel variants a batch launches: `variant` for members without the bedload
- * integral, `variant_sed` for members with it (unless set: the 50-row row blocks and
+ * integral, `variant_sed` for members with it (unless set: the 49-row row blocks and
  * the 8-row strips); takes effect at the next advance.  Members without the bedload
- * integral that are at least 2048 cells wide run the 50-row row blocks whatever
+ * integral that are at least 2048 cells wide run the 49-row row blocks whatever
  * `variant` says, as they would alone; the three groups launch side by side in the
  * batch's graph. */
 int rbs_batch_set_variant(rbs_batch *b, int32_t variant, int32_t variant_sed);

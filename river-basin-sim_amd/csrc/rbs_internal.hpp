@@ -220,8 +220,11 @@ inline size_t xf_rows(const rbs_state *s) { return (size_t)s->ny; }
 inline size_t yf_rows(const rbs_state *s) { return (size_t)s->ny + 1; }
 
 // Launch helpers implemented in the kernel translation units.
+// `ev_start` / `ev_stop`, when given, are recorded by the kernel's own dispatch (its
+// start and end), not by marker packets around it.
 int li_launch_step(rbs_state *s, bool device_scalars, double dt, double rain, int limit,
-                   double rain_scale);
+                   double rain_scale, hipEvent_t ev_start = nullptr,
+                   hipEvent_t ev_stop = nullptr);
 int reduce_speed_launch(rbs_state *s, int scheme, int mode);
 int reduce_partials_needed(const rbs_state *s);
 int finalize_launch(rbs_state *s, int mode, int scheme);

@@ -20,6 +20,8 @@
 //
 // Per block the kernel also emits fixed-order partials: max h (the next CFL dt),
 // sum f64(h) (ledger volume), infiltration and edge-drain depth removed.
+#include <hip/hip_ext.h>
+
 #include <cfloat>
 
 #include "rbs_device.hpp"
@@ -1403,7 +1405,8 @@ int li_num_blocks(const rbs_state *s) {
   return tx * ty;
 }
 
-int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
+int li_launch_step(rbs_state *s, bool, double, double, int limit, double, hipEvent_t ev_start,
+                   hipEvent_t ev_stop) {
   const LiVariant &v = variant_at(s->li_variant);
   LiArgs a{};
   li_fill_args(s, limit, &a);
@@ -1418,7 +1421,10 @@ int li_launch_step(rbs_state *s, bool, double, double, int limit, double) {
     fn = (a.flags & LI_SED) ? cv.kern_sed : cv.kern;
     threads = cv.threads;
   }
-  hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(threads), 0, s->stream, a);
+  if (ev_start || ev_stop)
+    hipExtLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(threads), 0, s->stream, ev_start, ev_stop, 0, a);
+  else
+    hipLaunchKernelGGL(fn, dim3(a.n_tiles), dim3(threads), 0, s->stream, a);
   RBS_HIP(hipGetLastError());
   s->cur ^= 1;
   return RBS_OK;

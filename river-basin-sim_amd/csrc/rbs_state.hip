@@ -767,10 +767,9 @@ int rbs_time_li_kernel(rbs_state *s, int32_t reps, double dt, double rain, doubl
   for (int k = 0; k < reps && rc == RBS_OK; ++k) {
     rc = set_scalars_launch(s, 0.0, dt, (float)rain, (float)rain_scale);
     if (rc) break;
-    if (hipEventRecord(ev[2 * k], s->stream) != hipSuccess) { rc = hip_fail(hipErrorUnknown, "hipEventRecord"); break; }
-    rc = li_launch_step(s, false, dt, rain, 1, rain_scale);
+    // the events are stamped by the kernel's dispatch itself (its start and end)
+    rc = li_launch_step(s, false, dt, rain, 1, rain_scale, ev[2 * k], ev[2 * k + 1]);
     if (rc) break;
-    if (hipEventRecord(ev[2 * k + 1], s->stream) != hipSuccess) { rc = hip_fail(hipErrorUnknown, "hipEventRecord"); break; }
     rc = finalize_launch(s, FIN_EXPLICIT, RBS_SCHEME_LI);
   }
   double total = 0.0;
