@@ -1254,25 +1254,33 @@ def hllc_sweep(st, dt: float, cells: int) -> list:
 def lake_at_rest(bed, dx: float, side: int = 1024, steps: int = 1000) -> dict:
     """Flat water 10 m above the lowest bed point of a central crop of the valley,
     closed walls: the largest velocity after ``steps`` HLLC steps (hydrostatic
-    balance over the channel banks and the floodplain relief)."""
-    from rbs.core import hllc
+    balance over the channel banks and the floodplain relief), with the bed as given
+    and with the run's datum shift ("auto": the whole metres below the lowest cell,
+    rbs.core.datum, as solver/run.py:194 applies it)."""
+    from rbs.core import datum, hllc
     from rbs.core.state import State
 
     n = bed.shape[0]
     r0 = (n - side) // 2
-    z = np.ascontiguousarray(bed[r0:r0 + side, r0:r0 + side])
-    eta = float(z.min()) + 10.0
-    st = State.from_bed(z, dx, depth=np.maximum(eta - z, 0.0).astype(np.float32), manning=0.035)
-    for _ in range(steps):
-        hllc.step(st, hllc.compute_dt(st, 0.45, 30.0))
-    u, v = st.velocities_numpy()
-    worst = float(max(np.abs(u).max(), np.abs(v).max()))
-    wet = float((st.h.numpy() > 1e-3).mean())
-    st.close()
-    log(f"[hllc] lake at rest {side}x{side}, {steps} steps, wet fraction {wet:.2f}: "
-        f"max|u,v| = {worst:.2e} m/s")
-    return {"grid": f"{side}x{side}", "steps": steps, "wet_fraction": round(wet, 3),
-            "max_velocity_m_s": worst}
+    z_raw = np.ascontiguousarray(bed[r0:r0 + side, r0:r0 + side])
+    out = {"grid": f"{side}x{side}", "steps": steps}
+    for key, z_ref in (("unshifted", 0.0), ("datum_shifted", datum.resolve_datum("auto", z_raw))):
+        z = datum.shift_bed(z_raw, z_ref)
+        eta = float(z.min()) + 10.0
+        st = State.from_bed(z, dx, depth=np.maximum(eta - z, 0.0).astype(np.float32),
+                            manning=0.035)
+        for _ in range(steps):
+            hllc.step(st, hllc.compute_dt(st, 0.45, 30.0))
+        u, v = st.velocities_numpy()
+        worst = float(max(np.abs(u).max(), np.abs(v).max()))
+        wet = float((st.h.numpy() > 1e-3).mean())
+        st.close()
+        log(f"[hllc] lake at rest {side}x{side} ({key}, z_ref {z_ref:.0f} m), {steps} steps, "
+            f"wet fraction {wet:.2f}: max|u,v| = {worst:.2e} m/s")
+        out[key] = {"z_ref_m": z_ref, "wet_fraction": round(wet, 3), "max_velocity_m_s": worst}
+    out["max_velocity_m_s"] = out["datum_shifted"]["max_velocity_m_s"]
+    out["reference_gate_m_s"] = 1e-4  # validation/test_hllc_2d.py:54
+    return out
 
 
 if __name__ == "__main__":

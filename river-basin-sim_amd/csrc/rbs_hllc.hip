@@ -584,12 +584,12 @@ __device__ __forceinline__ void hl_edge_face(int bc, bool first, float stage, fl
 // The neighbouring lane's value through a DPP wave-wide shift: lane i takes lane i-1's
 // (wave_shr:1) or lane i+1's (wave_shl:1); the lane with no neighbour keeps its own.
 __device__ __forceinline__ float hl_left(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                    0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138 /* wave_shr:1 */,
+                                                 0xF, 0xF, true));
 }
 __device__ __forceinline__ float hl_right(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                    0x130 /* wave_shl:1 */, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130 /* wave_shl:1 */,
+                                                 0xF, 0xF, true));
 }
 
 template <int STAGE, int H, int NW, bool SED>
@@ -810,22 +810,13 @@ __global__ __launch_bounds__(64 * NW) void hllc_strip_kernel(HlArgs a) {
         if (yh2 < 0.0f) q_out = q_out - yh2;
         if (yh1 > 0.0f) q_out = q_out + yh1;
         const float od = dtdx * q_out;
-        b2 = od > 0.0f ? fclamp(h2 / od, 0.0f, 1.0f) : 1.0f;
+        const bool divide = od > 0.0f && !(h2 >= od);
+        b2 = divide ? fclamp(h2 / od, 0.0f, 1.0f) : 1.0f;
         // _limit_fx on row R-2, _limit_fy on the face above it (:743-773)
         const float b2l = hl_left(b2);
-        xq = xh;
-        if (xq > 0.0f) {
-          if (c >= 1) xq = xq * b2l;
-        } else if (xq < 0.0f) {
-          if (c <= nx - 1) xq = xq * b2;
-        }
-        yq = yh2;
         const int f = R - 2;
-        if (yq > 0.0f) {
-          if (f >= 1) yq = yq * b3;
-        } else if (yq < 0.0f) {
-          if (f <= ny - 1) yq = yq * b2;
-        }
+        xq = xh * (xh > 0.0f ? (c >= 1 ? b2l : 1.0f) : (c <= nx - 1 ? b2 : 1.0f));
+        yq = yh2 * (yh2 > 0.0f ? (f >= 1 ? b3 : 1.0f) : (f <= ny - 1 ? b2 : 1.0f));
       }
 
       // row i = R-3: _accumulate + the RK stage (+ friction and sources on stage 2)

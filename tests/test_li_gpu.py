@@ -246,6 +246,38 @@ def test_odd_shapes_match(rbs, orc, shape):
     _cmp(g, o, tol=1e-6)
 
 
+@pytest.mark.parametrize("limit", [True, False])
+def test_friction_ratio_extremes_match_the_oracle(rbs, orc, limit):
+    """The face's friction quotient g dt n^2 |q| / h^float32(7/3) is the IEEE division
+    without its operand scaling, which acts only when the quotient vanishes in 1 + c/p
+    or |q| passes 1e19 m^2/s.  One step over discharges of 0, -0, subnormal and tiny
+    magnitudes up to 1e15 m^2/s, depths from 1e-3 m (the dry threshold, so every face
+    is wet) to 1e3 m and Manning n from 0 to 0.2, bitwise the oracle (whose division
+    is C's)."""
+    rng = np.random.default_rng(21)
+    ny, nx = 64, 96
+    mags = np.float32([0.0, 1e-45, 3e-42, 1e-38, 2e-30, 1e-20, 1e-10, 1e-4, 0.5, 30.0,
+                       1e3, 1e6, 1e9, 1e12, 1e15])
+    depths = np.float32([1e-3, 1.001e-3, 2e-3, 1e-2, 0.1, 1.0, 10.0, 100.0, 1e3])
+
+    def q_field(shape):
+        return (rng.choice(mags, shape) * rng.choice(np.float32([1.0, -1.0]), shape)).astype(
+            np.float32)
+
+    h = rng.choice(depths, (ny, nx)).astype(np.float32)
+    z = np.zeros((ny, nx), np.float32)
+    n = rng.choice(np.float32([0.0, 0.01, 0.05, 0.2]), (ny, nx)).astype(np.float32)
+    qx, qy = q_field((ny, nx + 1)), q_field((ny + 1, nx))
+    g, o = _pair(rbs, orc, z, 5.0, h, n)
+    g.qx, g.qy = qx, qy
+    o.qx[...] = qx
+    o.qy[...] = qy
+    rbs.li.step(g, 1e-3, limit=limit)
+    orc.li_step(o, 1e-3, limit=limit)
+    for a, b in ((g.qx.numpy(), o.qx), (g.qy.numpy(), o.qy), (g.h.numpy(), o.h)):
+        assert np.array_equal(a.view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
 def test_sources_sinks_open_edges_parity(rbs, orc):
     """Rain field (Kahan), infiltration, all edges open, on a sloped random bed."""
     rng = np.random.default_rng(5)
