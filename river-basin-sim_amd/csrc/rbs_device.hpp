@@ -160,8 +160,8 @@ __device__ __forceinline__ float div_uniform(float a, float b, float rb) {
   return normal ? q1 : a / b;
 }
 
-// c / p of the LI face's friction term (friction.py:31: g dt n^2 |q| / h^e), for c >= 0
-// and p = h^e with h >= kHDry.  The compiler's IEEE float division is this sequence
+// c / p of an LI face's friction term (friction.py:31: g dt n^2 |q| / h^e; the plain face
+// and a channel face's floodplain component), for c >= 0 and p = h^e with h >= kHDry.  The compiler's IEEE float division is this sequence
 // with v_div_scale around it, and v_div_scale changes nothing unless c < 2^-103, c/p
 // is subnormal or above 2^96, or p > 2^125.  The first two give c/p below 2^-80
 // (p >= 1e-7), where the division's last bits vanish in 1 + c/p = 1; the others need
@@ -246,7 +246,8 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
   const float h_fp = eta_max - z_bank;
   if (!(h_fp < kHDry)) {
     const float num = q_fp - g * h_fp * dt * slope;
-    fp = num / manning_den(q_fp, h_fp, 0.5f * (a.n + b.n), g, dt);
+    const float n_fp = 0.5f * (a.n + b.n);  // manning_den's expression, friction_ratio's quotient
+    fp = num / (1.0f + friction_ratio(g * dt * n_fp * n_fp * fabsf(q_fp), pow_e73(h_fp)));
   }
   const float w_face = fmn(a.w, b.w);
   float ch = 0.0f;
@@ -270,11 +271,11 @@ __device__ __forceinline__ float chan_face(const ChanCell &a, const ChanCell &b,
 // limit_qx_channels / limit_qy_channels (:432-484) for one face: a non-zero total
 // takes its donor's factor (lo: the left/top cell's, for positive flow; hi: the
 // right/bottom cell's, for negative), and so do both components.  `q` is the
-// unlimited total.
+// unlimited total.  A zero total's components are multiplied by 1 (the same bits),
+// so the face needs selects and no branch.
 __device__ __forceinline__ void limit_components(float q, float &q_ch, float &q_fp, float b_lo,
                                                  float b_hi) {
-  if (q == 0.0f) return;
-  const float b = q < 0.0f ? b_hi : b_lo;
+  const float b = q == 0.0f ? 1.0f : (q < 0.0f ? b_hi : b_lo);
   q_ch = q_ch * b;
   q_fp = q_fp * b;
 }

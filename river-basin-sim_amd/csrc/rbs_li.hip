@@ -1195,8 +1195,10 @@ __device__ __forceinline__ void li_chan_strip(const LiArgs &a, const int tile) {
     c1 = ChanCell{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};
     h1 = ph;
     ingest(y0 - 1);
-    // four rows per pass, as li_strip (348 -> 343 us at 4096^2)
-    _Pragma("unroll 4")
+    // two rows per pass: fewer spilled registers than four (the window renamed every
+    // second row; at 4096^2 349 -> 329 us, four rows per pass spill 60 B a lane, three
+    // 413 us)
+    _Pragma("unroll 2")
     for (int it = 1; it < H + 4; ++it) {
       const int R = y0 - 2 + it;
       const ChanCell c0{eta_subgrid_u(ph, pz, pw, pd, dx, rdx), pz, pn, pw, pd, pnc};
