@@ -6,6 +6,8 @@
 // kernels leave per-block partials, and finalize_kernel reduces them in a fixed
 // order, updates the Kahan-compensated float64 ledger (massbalance.py:39-158),
 // advances the float64 clock and -- in a device-driven run -- chooses the next dt.
+#include <type_traits>
+
 #include "rbs_device.hpp"
 
 namespace rbs {
@@ -133,8 +135,9 @@ __device__ void plan_next(DevCtl *ctl, const double *sync, double dx) {
 // mode: FIN_PRIME (no step was taken), FIN_EXPLICIT (host-driven step),
 // FIN_DRIVEN (device-driven step; plans the next one).
 // Thread t folds partials t, t + NT, t + 2 NT, ... in that order (sixteen loads per
-// array in flight per round: an 8192^2 HLLC state has 72 704 blocks), then the block
-// reduces the NT results in a fixed tree.
+// array in flight per round: an 8192^2 HLLC state has 72 704 blocks; then four, two
+// and one, so that a thread's last few partials do not wait for one another), then the
+// block reduces the NT results in a fixed tree.
 template <int NT>
 __device__ void finalize_body(DevCtl *ctl, Partials part, int nblk, double area, double n_cells,
                               const double *sync, double dx, int mode, int flip) {
@@ -142,34 +145,36 @@ __device__ void finalize_body(DevCtl *ctl, Partials part, int nblk, double area,
   // the partials are read before the run's state is known (reads only, in bounds), so
   // the two loads' latencies overlap
   const int active = ctl->active;
-  constexpr int U = 16;
   float m = 0.0f;
   double vol = 0.0, inf = 0.0, edge = 0.0;
   int b = threadIdx.x;
-  for (; b + (U - 1) * NT < nblk; b += U * NT) {
-    float mk[U];
-    double vk[U], ik[U], ek[U];
+  // U of the thread's partials loaded together, then folded in index order (the same
+  // order as one at a time): sixteen, four, two, one
+  auto fold = [&](auto u_tag) {
+    constexpr int U = decltype(u_tag)::value;
+    for (; b + (U - 1) * NT < nblk; b += U * NT) {
+      float mk[U];
+      double vk[U], ik[U], ek[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      mk[j] = part.speed_max[b + j * NT];
-      vk[j] = part.vol[b + j * NT];
-      ik[j] = part.infil[b + j * NT];
-      ek[j] = part.edge[b + j * NT];
-    }
+      for (int j = 0; j < U; ++j) {
+        mk[j] = part.speed_max[b + j * NT];
+        vk[j] = part.vol[b + j * NT];
+        ik[j] = part.infil[b + j * NT];
+        ek[j] = part.edge[b + j * NT];
+      }
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      m = fmx(m, mk[j]);
-      vol += vk[j];
-      inf += ik[j];
-      edge += ek[j];
+      for (int j = 0; j < U; ++j) {
+        m = fmx(m, mk[j]);
+        vol += vk[j];
+        inf += ik[j];
+        edge += ek[j];
+      }
     }
-  }
-  for (; b < nblk; b += NT) {
-    m = fmx(m, part.speed_max[b]);
-    vol += part.vol[b];
-    inf += part.infil[b];
-    edge += part.edge[b];
-  }
+  };
+  fold(std::integral_constant<int, 16>{});
+  fold(std::integral_constant<int, 4>{});
+  fold(std::integral_constant<int, 2>{});
+  fold(std::integral_constant<int, 1>{});
   if (mode == FIN_DRIVEN && active != 1) return;  // block-uniform
   block_reduce4<NT>(m, vol, inf, edge, s_red);
   if (threadIdx.x != 0) return;
