@@ -159,7 +159,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(a scenario shards by row strips; see DESIGN.md)", file=sys.stderr)
         return 2
 
@@ -459,7 +459,7 @@ def ensemble_figures(args) -> dict:
 
 
 def bench_ensemble(args) -> int:
-    """cfg4 as its own JSON line: scenarios/h of the batch (ensemble_figures)."""
+    """The 256-scenario ensemble as its own JSON line: scenarios/h of the batch (ensemble_figures)."""
     e = ensemble_figures(args)
     out = {
         "metric": METRIC,
@@ -528,7 +528,7 @@ def serve_figures(args, requests: int, capacity_sample: int) -> dict:
 
 
 def bench_serve(args) -> int:
-    """cfg5 as its own JSON line (serve_figures over ``--requests``)."""
+    """The serving loop as its own JSON line (serve_figures over ``--requests``)."""
     f = serve_figures(args, args.requests, args.capacity_sample)
     out = {
         "metric": METRIC,

@@ -1140,7 +1140,7 @@ static void hl_accumulate(orc_state *s, const float *h, float dx) {
 }
 
 /* hllc.py:1152-1269 _eval_L */
-/* The bedload integral of an HLLC stage (this build's coupling; the reference runs
+/* The bedload integral of an HLLC stage (a coupling the library adds; the reference runs
  * morphology on the local-inertial scheme only, config.py:451-461): every interior
  * face adds 0.5*dt times the MPM capacity of its limited mass flux at the LI face
  * depth max(eta) - max(z) of the stage state (sediment.py:161-273), so the two

@@ -15,7 +15,7 @@
 //   of every interior face's limited mass flux (the LI law and face depth,
 //   sediment.py:161-273), the RK2 average of the transport rate.  The reference
 //   runs morphology on the local-inertial scheme only (config.py:451-461); this
-//   coupling is this build's.
+//   coupling is the library's own addition.
 // A tile stages h, eta, z, u, v three cells out in LDS (the 4-point MUSCL stencil
 // of the ring-1 cells' faces), computes every face of the ring-1 cells (their
 // mass fluxes feed the donor beta the tile's own faces need), then works on its

@@ -57,7 +57,7 @@ def _cmp(g, o, tol=TOL):
     return dh, max(dqx, dqy)
 
 
-def _cfg1(rbs, orc):
+def _dambreak_512(rbs, orc):
     ny = nx = 512
     depth = np.where(np.arange(nx)[None, :] < 256, 1.0, 0.1) * np.ones((ny, 1))
     g, o = _pair(rbs, orc, np.zeros((ny, nx)), 1.0, depth, 0.01)
@@ -66,7 +66,7 @@ def _cfg1(rbs, orc):
 
 
 def _stoker_score(orc, row, t):
-    """validation/test_dam_break.py:93-133's scores on the cfg1 centreline row."""
+    """validation/test_dam_break.py:93-133's scores on the 512^2 dam break's centreline row."""
     nx = row.size
     x = (np.arange(nx) + 0.5 - 256.0) * 1.0
     h_ref, _ = orc.stoker(x, t, 1.0, 0.1)
@@ -78,14 +78,14 @@ def _stoker_score(orc, row, t):
     return nrmse, abs(x[h > lvl].max() - fref) / fref
 
 
-def test_cfg1_dambreak_1000_steps_parity_mass_and_stoker(rbs, orc):
+def test_dambreak_512_1000_steps_parity_mass_and_stoker(rbs, orc):
     """512x512 dam break (h=1 for x<256, 0.1 else), n=0.01, closed, the fixed dt
     alpha dx / sqrt(g h_max(t=0)) for 1000 steps.  The wave reflections raise h_max to
     1.65 m, above which this dt exceeds the CFL bound (about step 100 on), so the run
     is sensitive to every last bit: the HIP path equals the default oracle bit for bit,
     and the libm oracle (the reference's glibc powf) as well -- no power in this run
     lands on one of glibc's misrounded inputs."""
-    g, o, o_libm = _cfg1(rbs, orc)
+    g, o, o_libm = _dambreak_512(rbs, orc)
     dt = 0.7 * 1.0 / math.sqrt(9.81 * 1.0)  # compute_dt at t = 0 (h_max = 1)
     assert rbs.li.compute_dt(g, 0.7, 30.0) == dt == orc.li_compute_dt(o, 0.7, 30.0)
     led = rbs.MassLedger.from_state(g)
@@ -107,16 +107,16 @@ def test_cfg1_dambreak_1000_steps_parity_mass_and_stoker(rbs, orc):
     dh_l, dq_l = _cmp(g, o_libm)
     rec = led.record(g, 1000 * dt)
     assert rec.rel_error < 1e-7, f"mass closure {rec.rel_error:.2e}"
-    print(f"\n[cfg1 parity] default oracle max|dh|={dh:.2e} max|dq|={dq:.2e}; libm oracle "
+    print(f"\n[dam break 512^2 parity] default oracle max|dh|={dh:.2e} max|dq|={dq:.2e}; libm oracle "
           f"{dh_l:.2e} / {dq_l:.2e}; mass={rec.rel_error:.2e} stoker nRMSE={nrmse:.4f} "
           f"front={front_err:.4f}")
 
 
-def test_cfg1_at_a_stable_dt(rbs, orc):
-    """cfg1 at a fixed dt that keeps the CFL bound for all 1000 steps: alpha dx /
+def test_dambreak_512_at_a_stable_dt(rbs, orc):
+    """The 512^2 dam break at a fixed dt that keeps the CFL bound for all 1000 steps: alpha dx /
     sqrt(g h_peak) with h_peak = 2 m above the 1.65 m the reflections reach.  Parity with
     both oracle modes; the Stoker scores at the same time as the fixed-dt run, reported."""
-    g, o, o_libm = _cfg1(rbs, orc)
+    g, o, o_libm = _dambreak_512(rbs, orc)
     dt = 0.7 * 1.0 / math.sqrt(9.81 * 2.0)
     k_stoker = int(round(250 * 0.7 / math.sqrt(9.81) / dt))  # t = 55.9 s
     for k in range(k_stoker):
@@ -133,7 +133,7 @@ def test_cfg1_at_a_stable_dt(rbs, orc):
     # the LI scheme drops advection and lags this strong bore (h_R/h_L = 0.1): 0.187 and
     # 0.180 here; the reference's band (0.10, 0.15) is for its 0.3 bore, passed above
     assert nrmse < 0.25 and front_err < 0.25
-    print(f"\n[cfg1 stable dt {dt:.4f} s] default oracle {dh:.2e} / {dq:.2e}; libm oracle "
+    print(f"\n[dam break 512^2, stable dt {dt:.4f} s] default oracle {dh:.2e} / {dq:.2e}; libm oracle "
           f"{dh_l:.2e} / {dq_l:.2e}; stoker nRMSE={nrmse:.4f} front={front_err:.4f}")
 
 
@@ -145,7 +145,7 @@ def basin4096():
     return terrain.rain_on_terrain(4096, seed=20261203)
 
 
-def test_cfg2_crop_1000_fixed_dt_steps(rbs, orc, basin4096):
+def test_rain_basin_crop_512_1000_fixed_dt_steps(rbs, orc, basin4096):
     """A 512^2 crop of the rain-on-terrain basin's generators (its centre), all edges
     open, rain field with Kahan debt, infiltration, initially dry, 1000 steps of 1 s:
     bitwise the default oracle (h, qx, qy and the Kahan debt) and within 1e-5 of the
@@ -169,11 +169,11 @@ def test_cfg2_crop_1000_fixed_dt_steps(rbs, orc, basin4096):
     dh_l, dq_l = _cmp(g, o_libm)
     wet = int((g.h.numpy() >= 1e-3).sum())
     assert wet > 50_000
-    print(f"\n[cfg2 crop] default oracle {dh:.2e} / {dq:.2e}; libm oracle {dh_l:.2e} / "
+    print(f"\n[rain basin 512^2 crop] default oracle {dh:.2e} / {dq:.2e}; libm oracle {dh_l:.2e} / "
           f"{dq_l:.2e}; wet cells {wet}")
 
 
-def test_cfg2_full_size_device_driven(rbs, basin4096):
+def test_rain_basin_4096_device_driven(rbs, basin4096):
     """The whole 4096^2 basin, device-driven (the headline kernel, as the bench runs it)
     from dry through 1800 s of storm, several hundred steps: mass closes within 1e-7 of the
     throughput (inflow + outflow), every depth is finite and non-negative."""
@@ -201,7 +201,7 @@ def test_cfg2_full_size_device_driven(rbs, basin4096):
     inflow, outflow = led.rain_in, led.infil_out + led.edge_out
     closure = abs(inflow - outflow - led.volume) / (inflow + outflow)
     assert closure <= 1e-7, closure
-    print(f"\n[cfg2 4096^2] {steps} steps to 1800 s, mass closure {closure:.2e} of throughput, "
+    print(f"\n[rain basin 4096^2] {steps} steps to 1800 s, mass closure {closure:.2e} of throughput, "
           f"h_max {h.max():.3f} m")
 
 

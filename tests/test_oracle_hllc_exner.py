@@ -1,4 +1,4 @@
-"""HLLC + Exner in the oracle: the coupling this build adds (the reference runs
+"""HLLC + Exner in the oracle: the coupling the library adds (the reference runs
 morphology on the local-inertial scheme only, config.py:451-461), pinned by the
 invariants the reference's morphology gates rest on rather than by a reference run:
 

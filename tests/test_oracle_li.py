@@ -83,7 +83,7 @@ def test_stoker_star_depth_closed_form(orc):
     u_rare = 2.0 * (math.sqrt(g) - math.sqrt(g * hm))
     u_shock = (hm - 0.1) * math.sqrt(0.5 * g * (hm + 0.1) / (hm * 0.1))
     assert abs(u_rare - u_shock) < 1e-9
-    assert hm == pytest.approx(0.39617, abs=1e-5)  # the cfg1 star depth (h_L=1, h_R=0.1)
+    assert hm == pytest.approx(0.39617, abs=1e-5)  # the 512^2 dam break's star depth (h_L=1, h_R=0.1)
 
 
 def test_rest_state_stays_at_rest(orc):

@@ -2,7 +2,7 @@
 solver/io/test_viewer_export.py), with frames from oracle runs.
 
 zarr and xarray are not installed here, so the store is checked against the Zarr v3
-format itself (metadata documents, chunk keys, chunk bytes) through this build's
+format itself (metadata documents, chunk keys, chunk bytes) through the library's own
 reader; that zarr-python opens it is unpinned.
 """
 
