@@ -160,7 +160,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
-              "(a scenario shards by row strips; see DESIGN.md)", file=sys.stderr)
+              "(a scenario would shard by row strips, one per GPU)", file=sys.stderr)
         return 2
 
     from rbs import _lib
