@@ -513,10 +513,11 @@ __device__ __forceinline__ float (&strip_lds_rows())[SLOTS][4][BWC] {
   return s_row;
 }
 
-// LIM = 1: the limiter is on (the flag known at compile time, so that the compiler
-// does not rebuild the flag's lane mask with vector instructions every row: 162.9 ->
-// 161.4 us at 4096^2); LIM = 0 reads it from the flags.
-template <int H, bool SED, int NW = 0, int FL = 1, int LIM = 0>
+// LIM = 1: the limiter is on, CMP = 1: the Kahan debt is armed (flags known at compile
+// time, so that the compiler does not rebuild their lane masks with vector instructions
+// every row: at 4096^2 162.9 -> 161.4 us for the limiter, 160.7 -> 158.8 us for the
+// debt); 0 reads them from the flags.
+template <int H, bool SED, int NW = 0, int FL = 1, int LIM = 0, int CMP = 0>
 __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
   // FL: finished rows stored per barrier, from a ring of 2 FL LDS rows (two rows per
   // barrier measured 3 % faster than one at 4096^2; three no better, four cost
@@ -547,7 +548,7 @@ __device__ __forceinline__ void li_strip(const LiArgs &a, const int tile) {
 
   const int ny = a.ny, nx = a.nx;
   const int64_t P = a.pitch;
-  const int flags = a.flags;
+  const int flags = CMP ? (a.flags | LI_COMP) : a.flags;  // CMP = 1: Kahan debt armed
   const bool limit = LIM ? true : (flags & LI_LIMIT) != 0;
   const StepScalars k = step_scalars(a, ctl);
   const float dx = (float)a.dx64;
@@ -1007,9 +1008,10 @@ template <int H, bool SED, int W, int NW = 0, int FL = 1>
 __global__ __launch_bounds__(NW > 0 ? 64 * NW : kThreads)
 __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_kernel(LiArgs a) {
   const int tile = xcd_tile(blockIdx.x, a.n_tiles);
-  if constexpr (NW > 0 && !SED) {  // row blocks: the limiter's flag as a constant
+  if constexpr (NW > 0 && !SED) {  // row blocks: the limiter's and debt's flags as constants
     if (a.flags & LI_LIMIT) {
-      li_strip<H, SED, NW, FL, 1>(a, tile);
+      if (a.flags & LI_COMP) li_strip<H, SED, NW, FL, 1, 1>(a, tile);
+      else li_strip<H, SED, NW, FL, 1>(a, tile);
       return;
     }
   }
@@ -1052,7 +1054,8 @@ __attribute__((amdgpu_waves_per_eu(SED ? 4 : W))) void li_strip_batch_kernel(
   if (batch_tile(tile_member, tile_first, dims, seg, m, tile)) {
     if constexpr (NW > 0 && !SED) {  // as li_strip_kernel
       if (members[m].flags & LI_LIMIT) {
-        li_strip<H, SED, NW, FL, 1>(members[m], tile);
+        if (members[m].flags & LI_COMP) li_strip<H, SED, NW, FL, 1, 1>(members[m], tile);
+        else li_strip<H, SED, NW, FL, 1>(members[m], tile);
         return;
       }
     }
